@@ -1,0 +1,81 @@
+// biwfa_device.hpp -- shared host/device definitions of the gfx950 BiWFA kernels.
+#pragma once
+#include <stdint.h>
+
+namespace bw {
+
+// Offset value of an absent diagonal (WFA2-lib wavefront_offset.h:44).
+constexpr int32_t kOffsetNull = (int32_t)(-2147483647 - 1) / 2;
+
+// BiWFA thresholds (wavefront_bialign.c:48-50).
+constexpr int kFallbackMinScore = 250;
+constexpr int kFallbackMinLength = 100;
+constexpr int kRecoveryMinScore = 500;
+
+// Internal status codes (wfa.h:52-55) and device-only ones.
+constexpr int kStOk = -1;
+constexpr int kStEndReached = -2;
+constexpr int kStEndUnreachable = -3;
+constexpr int kStMaxSteps = -100;
+constexpr int kStUnattainable = -300;
+constexpr int kStCapacity = -400;
+constexpr int kStGuard = -405;       // termination guard tripped (never in a correct run)
+
+constexpr int kMaxScope = 32;          // max(o+e, x)+1 supported by the ring layout
+constexpr int kWave = 64;              // gfx950 wavefront size
+constexpr int kWavesPerBlock = 4;      // independent waves per 256-thread workgroup
+
+// Per-wave workspace layout, in int32 units from the wave's workspace base.
+struct WsLayout {
+  int ring_hdr;     // 2 dirs x scope x 3 comps x {lo, hi, present, pad}
+  int data;         // ring data (2 x scope x 3 x wcap)  UNION  base-case arena
+  int wcap;         // ring width capacity (power of two)
+  int arena_cap;    // ints available at `data` for the base-case arena
+  int bhdr;         // base-case headers: hcap scores x 3 comps x {lo, hi, kbase}
+  int hcap;         // max base-case score + 1
+  int runs_tmp;     // reversed runs of one backtrace (uint32)
+  int rcap;
+  int stack;        // recursion stack: scap entries x 8 ints
+  int scap;
+  int seq;          // staged 2-bit sequences (LDS path): P then T, each with pad words
+  int seq_words;    // words available for each of P and T (including 3 pad words)
+  int total;        // ints per wave
+};
+
+// Kernel parameters (passed by value).
+struct KParams {
+  // input batch (device)
+  const uint64_t* words;
+  const int64_t* p_woff;
+  const int32_t* p_len;
+  const int64_t* t_woff;
+  const int32_t* t_len;
+  // work source: order[work_list ? work_list[i] : i] for i < count
+  const int32_t* order;
+  const int32_t* work_list;      // optional indirection (spill list)
+  const int32_t* work_count;     // device count (if non-null) else num_work
+  int32_t num_work;
+  int32_t* queue;                // [0] = next work index (atomic)
+  int32_t chunk;
+  // spill output (LDS kernel)
+  int32_t* spill_count;
+  int32_t* spill_list;
+  // penalties: internal (after the match<0 transform) and user (for the .alg score)
+  int32_t x, o, e, scope;
+  int32_t um, ux, uo, ue;
+  int32_t score_only;
+  int32_t max_steps;
+  // outputs (device, indexed by pair id)
+  int32_t* score;
+  int32_t* wf_score;
+  int32_t* status;
+  uint32_t* runs;
+  const int64_t* runs_off;       // [N+1] run capacity prefix
+  int32_t* nruns;
+  // workspace
+  int32_t* gws;                  // global workspace (global kernel)
+  int64_t gws_stride;            // ints per wave
+  WsLayout L;
+};
+
+}  // namespace bw

@@ -1,0 +1,821 @@
+// biwfa_kernel.hip -- gfx950 bidirectional WFA (score + CIGAR), one wave64 per pair.
+//
+// Behaviour follows WFA2-lib v2.3 `wavefront_align` in exact BiWFA mode
+// (memory_mode = ultralow, heuristic = none, gap-affine):
+//   recursion driver      wavefront/wavefront_bialign.c:581-658, 703-730
+//   breakpoint search     wavefront/wavefront_bialign.c:411-519
+//   overlap detection     wavefront/wavefront_bialign.c:189-392
+//   affine compute        wavefront/wavefront_compute_affine.c:44-86, 229-260
+//                         wavefront/wavefront_compute.c:40-104, 306-352, 382-632
+//   extend / termination  wavefront/wavefront_extend.c:86-211, wavefront_termination.c:37-114
+//   base case + backtrace wavefront/wavefront_bialign.c:155-185,
+//                         wavefront/wavefront_unialign.c:54-273,
+//                         wavefront/wavefront_backtrace.c:38-529
+//
+// Device mapping:
+//   * A persistent grid of 256-thread workgroups; each of the 4 waves is an
+//     independent aligner that pulls pairs from a device work queue (atomic
+//     counter over a schedule order).  No cross-wave synchronisation.
+//   * Diagonals are spread over the 64 lanes; wavefront limits, scores and the
+//     recursion stack are wave-uniform.
+//   * Forward and reverse wavefronts live in modular rings (scope scores x
+//     M/I1/D1 x wcap diagonals, k indexed modulo wcap) -- O(s) memory per pair.
+//     The base case keeps every wavefront of its sub-problem in a bump arena
+//     that reuses the ring storage.
+//   * Sequences are 2-bit packed; extension compares 32 bases per 64-bit word
+//     (xor + count-trailing-zeros forward, count-leading-zeros backward).
+//   * Two instantiations: kLds keeps the whole workspace and both sequences in
+//     LDS; a pair that outgrows any LDS capacity is pushed to a spill list and
+//     re-run by the global-workspace instantiation with large capacities.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include "biwfa_device.hpp"
+
+namespace bw {
+
+#define DI __device__ __forceinline__
+
+enum { CM = 0, CI = 1, CD = 2 };
+constexpr int kAbsent = INT_MIN;   // base-case header: component not allocated
+// uint32 run op codes (SAM encoding with mismatches shown, cigar.c:38-56)
+constexpr int kOpM = 7, kOpX = 8, kOpI = 1, kOpD = 2;
+
+DI int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+DI int imax(int a, int b) { return a >= b ? a : b; }
+DI int imin(int a, int b) { return a <= b ? a : b; }
+
+// Orders this wave's memory writes before its later reads by other lanes.
+DI void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+DI int wave_max(int v) {
+  for (int d = 32; d >= 1; d >>= 1) v = imax(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+// A wavefront component as seen by readers: offsets for k in [lo,hi] live at
+// ws[base + (k & mask)] (mask = wcap-1 in a ring, -1 in the arena).  lo > hi is
+// the null wavefront (WFA2's wavefront_null has lo=1, hi=-1).
+struct Wf {
+  int lo, hi, base, mask;
+};
+DI Wf wf_null() { return Wf{1, -1, 0, 0}; }
+DI int rd(const int32_t* ws, const Wf& w, int k) {
+  return (k >= w.lo && k <= w.hi) ? ws[w.base + (k & w.mask)] : kOffsetNull;
+}
+
+// One aligner's view of a sub-problem.
+struct Dir {
+  int pb, plen, tb, tlen;   // sub-range of the pair
+  int rev;                  // 1: reads the sub-range backwards (reverse aligner)
+  int ring;                 // 0 forward ring, 1 reverse ring, -1 base-case arena
+  int cb, ce;               // begin / end component
+  int null_steps;
+  int status, status_score;
+};
+
+struct BP {                 // wavefront_bialigner.h:43-54
+  int score, sf, sr, kf, kr, of, orr, comp;
+};
+
+struct Ctx {
+  int32_t* ws;
+  const uint64_t* P;
+  const uint64_t* T;
+  int lane;
+  WsLayout L;
+  int x, o, e, scope, max_steps;
+  int um, ux, uo, ue;
+  int arena_top;
+  uint32_t* out;
+  int out_cap, out_n, pend_op, pend_len, alg_score;
+  int spill;      // a workspace capacity was exceeded
+  int cap_err;    // the pair's output run capacity was exceeded
+  int guard;      // score bound of the current sub-problem (termination guard)
+};
+
+// Upper bound of the optimal score of a plen x tlen sub-problem plus the
+// slack the breakpoint search may run past it; a search that goes beyond it
+// can only be a defect, and is stopped with kStGuard instead of looping.
+DI int score_guard(const Ctx& c, int plen, int tlen) {
+  const int mn = imin(plen, tlen), df = imax(plen, tlen) - mn;
+  return mn * c.x + c.o + df * c.e + c.o + 2 * c.scope + 16;
+}
+
+// ---------------------------------------------------------------------------
+// storage addressing
+
+DI int ring_hdr(const Ctx& c, int ring, int slot, int comp) {
+  return c.L.ring_hdr + ((ring * c.scope + slot) * 3 + comp) * 4;
+}
+DI int ring_base(const Ctx& c, int ring, int slot, int comp) {
+  return c.L.data + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
+}
+DI int base_hdr(const Ctx& c, int score, int comp) { return c.L.bhdr + (score * 3 + comp) * 3; }
+
+// Component `comp` at `score` (WFA2's wavefront_compute_get_*wavefront,
+// wavefront_compute.c:266-305).  *present reports a non-NULL WFA2 pointer
+// (possibly with lo > hi); the returned view is null when absent or empty.
+DI Wf get_wf(const Ctx& c, const Dir& d, int score, int comp, int* present) {
+  *present = 0;
+  if (score < 0) return wf_null();
+  if (d.ring >= 0) {
+    const int slot = score % c.scope;
+    const int h = ring_hdr(c, d.ring, slot, comp);
+    if (!uni(c.ws[h + 2])) return wf_null();
+    *present = 1;
+    const int lo = uni(c.ws[h]), hi = uni(c.ws[h + 1]);
+    if (lo > hi) return Wf{lo, hi, 0, 0};
+    return Wf{lo, hi, ring_base(c, d.ring, slot, comp), c.L.wcap - 1};
+  }
+  const int h = base_hdr(c, score, comp);
+  const int kb = uni(c.ws[h + 2]);
+  if (kb == kAbsent) return wf_null();
+  *present = 1;
+  return Wf{uni(c.ws[h]), uni(c.ws[h + 1]), kb, -1};
+}
+DI Wf get_in(const Ctx& c, const Dir& d, int score, int comp) {
+  int present;
+  Wf w = get_wf(c, d, score, comp, &present);
+  if (w.lo > w.hi) return wf_null();
+  return w;
+}
+DI void set_hdr(Ctx& c, const Dir& d, int score, int comp, int lo, int hi, int base_or_absent,
+                int present) {
+  if (d.ring >= 0) {
+    const int h = ring_hdr(c, d.ring, score % c.scope, comp);
+    c.ws[h] = lo;
+    c.ws[h + 1] = hi;
+    c.ws[h + 2] = present;
+  } else {
+    const int h = base_hdr(c, score, comp);
+    c.ws[h] = lo;
+    c.ws[h + 1] = hi;
+    c.ws[h + 2] = present ? base_or_absent : kAbsent;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sequences: 2-bit packed, base j at bits 2*(j%32) of word j/32
+
+DI uint64_t window(const uint64_t* s, int pos) {       // bases [pos, pos+32)
+  const int wi = pos >> 5;
+  const int sh = (pos & 31) << 1;
+  const uint64_t lo = s[wi], hi = s[wi + 1];
+  return (lo >> sh) | ((hi << 1) << (63 - sh));
+}
+
+// Longest common extension from (v,h) of the aligner's sub-problem, 32 bases
+// per 64-bit compare, clamped to the sub-problem's end.
+DI int lcp(const Ctx& c, const Dir& d, int v, int h) {
+  const int lim = imin(d.plen - v, d.tlen - h);
+  int n = 0;
+  if (!d.rev) {
+    const int pp = d.pb + v, tp = d.tb + h;
+    while (n < lim) {
+      const uint64_t x = window(c.P, pp + n) ^ window(c.T, tp + n);
+      if (x) { n += __builtin_ctzll(x) >> 1; break; }
+      n += 32;
+    }
+  } else {
+    // walking down from P[pe-1-v], T[te-1-h]: windows end at the current base
+    const int pp = d.pb + d.plen - 1 - v, tp = d.tb + d.tlen - 1 - h;
+    while (n < lim) {
+      const uint64_t x = window(c.P, pp - n - 31) ^ window(c.T, tp - n - 31);
+      if (x) { n += __builtin_clzll(x) >> 1; break; }
+      n += 32;
+    }
+  }
+  return imin(n, lim);
+}
+
+// ---------------------------------------------------------------------------
+// compute: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
+
+DI void compute_step(Ctx& c, Dir& d, int s) {
+  const Wf mm = get_in(c, d, s - c.x, CM);
+  const Wf mo = get_in(c, d, s - c.o - c.e, CM);
+  const Wf ie = get_in(c, d, s - c.e, CI);
+  const Wf de = get_in(c, d, s - c.e, CD);
+  if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return; }
+  const bool nmm = mm.lo > mm.hi, nmo = mo.lo > mo.hi, nie = ie.lo > ie.hi, nde = de.lo > de.hi;
+  if (nmm && nmo && nie && nde) {               // null step
+    d.null_steps++;
+    set_hdr(c, d, s, CM, 1, -1, 0, 0);
+    set_hdr(c, d, s, CI, 1, -1, 0, 0);
+    set_hdr(c, d, s, CD, 1, -1, 0, 0);
+    wave_sync();
+    return;
+  }
+  d.null_steps = 0;
+  // wavefront_compute_limits_input (wavefront_compute.c:40-86)
+  int lo = mm.lo, hi = mm.hi;
+  lo = imin(lo, mo.lo - 1); hi = imax(hi, mo.hi + 1);
+  lo = imin(lo, ie.lo + 1); hi = imax(hi, ie.hi + 1);
+  lo = imin(lo, de.lo - 1); hi = imax(hi, de.hi - 1);
+  const bool has_i = !nmo || !nie, has_d = !nmo || !nde;
+  const int width = hi - lo + 1;
+  int bm, bi, bd, mask;
+  if (d.ring >= 0) {
+    if (width > c.L.wcap) { c.spill = 1; return; }
+    const int slot = s % c.scope;
+    bm = ring_base(c, d.ring, slot, CM);
+    bi = ring_base(c, d.ring, slot, CI);
+    bd = ring_base(c, d.ring, slot, CD);
+    mask = c.L.wcap - 1;
+  } else {
+    const int need = width * (1 + (int)has_i + (int)has_d);
+    if (c.arena_top + need > c.L.arena_cap) { c.spill = 1; return; }
+    int top = c.L.data + c.arena_top;
+    bm = top - lo; top += width;
+    bi = top - lo; if (has_i) top += width;
+    bd = top - lo; if (has_d) top += width;
+    c.arena_top = top - c.L.data;
+    mask = -1;
+  }
+  const uint32_t tl = (uint32_t)d.tlen, pl = (uint32_t)d.plen;
+  int fm = INT_MAX, lm = INT_MIN, fi = INT_MAX, li = INT_MIN, fd = INT_MAX, ld = INT_MIN;
+  int32_t* const ws = c.ws;
+  for (int kb = lo; kb <= hi; kb += kWave) {
+    const int k = kb + c.lane;
+    const bool act = k <= hi;
+    // wavefront_compute_affine_idm (wavefront_compute_affine.c:65-85)
+    const int ins1 = imax(rd(ws, mo, k - 1), rd(ws, ie, k - 1)) + 1;
+    const int del1 = imax(rd(ws, mo, k + 1), rd(ws, de, k + 1));
+    const int misms = rd(ws, mm, k) + 1;
+    int mx = imax(del1, imax(misms, ins1));
+    if ((uint32_t)mx > tl || (uint32_t)(mx - k) > pl) mx = kOffsetNull;
+    if (act) {
+      ws[bm + (k & mask)] = mx;
+      if (has_i) ws[bi + (k & mask)] = ins1;
+      if (has_d) ws[bd + (k & mask)] = del1;
+    }
+    // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
+    const uint64_t b_m = __ballot(act && mx != kOffsetNull);
+    const uint64_t b_i = __ballot(act && (uint32_t)ins1 <= tl && (uint32_t)(ins1 - k) <= pl);
+    const uint64_t b_d = __ballot(act && (uint32_t)del1 <= tl && (uint32_t)(del1 - k) <= pl);
+    if (b_m) { if (fm == INT_MAX) fm = kb + __builtin_ctzll(b_m); lm = kb + 63 - __builtin_clzll(b_m); }
+    if (b_i) { if (fi == INT_MAX) fi = kb + __builtin_ctzll(b_i); li = kb + 63 - __builtin_clzll(b_i); }
+    if (b_d) { if (fd == INT_MAX) fd = kb + __builtin_ctzll(b_d); ld = kb + 63 - __builtin_clzll(b_d); }
+  }
+  // trimmed limits: hi = last in-bounds (else lo-1), lo = first in-bounds (else lo)
+  if (lm == INT_MIN) { fm = lo; lm = lo - 1; }
+  if (li == INT_MIN) { fi = lo; li = lo - 1; }
+  if (ld == INT_MIN) { fd = lo; ld = lo - 1; }
+  set_hdr(c, d, s, CM, fm, lm, bm, 1);
+  set_hdr(c, d, s, CI, fi, li, bi, has_i);
+  set_hdr(c, d, s, CD, fd, ld, bd, has_d);
+  wave_sync();
+}
+
+// ---------------------------------------------------------------------------
+// extend + termination (wavefront_extend.c:86-211, wavefront_termination.c:37-114)
+
+DI bool terminated(const Ctx& c, const Dir& d, int s) {
+  const int k_end = d.tlen - d.plen;
+  int present;
+  const Wf w = get_wf(c, d, s, d.ce, &present);
+  if (!present || w.lo > k_end || k_end > w.hi) return false;
+  return uni(c.ws[w.base + (k_end & w.mask)]) >= d.tlen;
+}
+
+// Returns 1 when the aligner is done; *max_ak gets the max antidiagonal.
+DI int extend_step(Ctx& c, Dir& d, int s, int* max_ak) {
+  *max_ak = 0;
+  int present;
+  const Wf m = get_wf(c, d, s, CM, &present);
+  if (!present) {
+    if (d.null_steps > c.scope) {
+      d.status = kStEndUnreachable;
+      d.status_score = s;
+      return 1;
+    }
+    return 0;
+  }
+  int lmax = 0;
+  for (int kb = m.lo; kb <= m.hi; kb += kWave) {
+    const int k = kb + c.lane;
+    if (k <= m.hi) {
+      const int a = m.base + (k & m.mask);
+      int off = c.ws[a];
+      if (off != kOffsetNull) {
+        off += lcp(c, d, off - k, off);
+        c.ws[a] = off;
+        lmax = imax(lmax, 2 * off - k);
+      }
+    }
+  }
+  wave_sync();
+  lmax = uni(wave_max(lmax));
+  if (terminated(c, d, s)) {
+    d.status = kStEndReached;
+    d.status_score = s;
+    return 1;
+  }
+  *max_ak = lmax;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// breakpoint detection (wavefront_bialign.c:189-392)
+
+// First k0 (ascending) of the overlapping interval that satisfies the overlap
+// condition; component `comp` (CD/CI: indel-to-indel, CM: M-to-M).
+DI void scan_breakpoint(const Ctx& c, bool forward, int s0, int s1, const Wf& w0, const Wf& w1,
+                        int comp, int tl, int pl, BP& bp) {
+  const int lo0 = w0.lo, hi0 = w0.hi;
+  const int lo1 = tl - pl - w1.hi, hi1 = tl - pl - w1.lo;
+  if (hi1 < lo0 || hi0 < lo1) return;
+  const int min_hi = imin(hi0, hi1), max_lo = imax(lo0, lo1);
+  const int sc = (comp == CM) ? s0 + s1 : s0 + s1 - c.o;
+  if (!(sc < bp.score)) return;
+  for (int kb = max_lo; kb <= min_hi; kb += kWave) {
+    const int k0 = kb + c.lane;
+    const int k1 = tl - pl - k0;
+    bool ok = false;
+    int o0 = 0, o1 = 0;
+    if (k0 <= min_hi) {
+      o0 = c.ws[w0.base + (k0 & w0.mask)];
+      o1 = c.ws[w1.base + (k1 & w1.mask)];
+      ok = o0 + o1 >= tl;
+      if (ok && comp != CM) {   // indel: forward coordinates must be in bounds (:226-240)
+        ok = forward ? !(o0 - k0 > pl || o0 > tl) : !(o1 - k1 > pl || o1 > tl);
+      }
+    }
+    const uint64_t b = __ballot(ok);
+    if (b) {
+      const int j = __builtin_ctzll(b);
+      const int a0 = uni(__shfl(o0, j, kWave)), a1 = uni(__shfl(o1, j, kWave));
+      const int kk0 = kb + j, kk1 = tl - pl - kk0;
+      if (forward) {
+        bp.sf = s0; bp.sr = s1; bp.kf = kk0; bp.kr = kk1; bp.of = a0; bp.orr = a1;
+      } else {
+        bp.sf = s1; bp.sr = s0; bp.kf = kk1; bp.kr = kk0; bp.of = a1; bp.orr = a0;
+      }
+      bp.score = sc;
+      bp.comp = comp;
+      return;
+    }
+  }
+}
+
+DI void overlap(const Ctx& c, const Dir& a0, const Dir& a1, int s0, int s1, bool forward,
+                BP& bp) {
+  int pm0, pi0, pd0;
+  const Wf m0 = get_wf(c, a0, s0, CM, &pm0);
+  if (!pm0) return;
+  const Wf i0 = get_wf(c, a0, s0, CI, &pi0);
+  const Wf d0 = get_wf(c, a0, s0, CD, &pd0);
+  const int tl = a0.tlen, pl = a0.plen;
+  for (int i = 0; i < c.scope; ++i) {
+    const int si = s1 - i;
+    if (si < 0) break;
+    if (s0 + si - c.o >= bp.score) continue;
+    int p1;
+    const Wf d1 = get_wf(c, a1, si, CD, &p1);
+    if (pd0 && p1) scan_breakpoint(c, forward, s0, si, d0, d1, CD, tl, pl, bp);
+    const Wf i1 = get_wf(c, a1, si, CI, &p1);
+    if (pi0 && p1) scan_breakpoint(c, forward, s0, si, i0, i1, CI, tl, pl, bp);
+    if (s0 + si >= bp.score) continue;
+    const Wf m1 = get_wf(c, a1, si, CM, &p1);
+    if (p1) scan_breakpoint(c, forward, s0, si, m0, m1, CM, tl, pl, bp);
+  }
+}
+
+// wavefront_aligner_init_wf (wavefront_aligner.c:311-383) for a ring aligner.
+DI void ring_init(Ctx& c, Dir& d) {
+  for (int i = c.lane; i < c.scope * 3; i += kWave) {
+    const int h = c.L.ring_hdr + (d.ring * c.scope * 3 + i) * 4;
+    c.ws[h + 2] = 0;
+  }
+  wave_sync();
+  const int h = ring_hdr(c, d.ring, 0, d.cb);
+  c.ws[h] = 0;
+  c.ws[h + 1] = 0;
+  c.ws[h + 2] = 1;
+  c.ws[ring_base(c, d.ring, 0, d.cb)] = 0;
+  d.null_steps = 0;
+  d.status = kStOk;
+  d.status_score = 0;
+  wave_sync();
+}
+
+// wavefront_bialign_find_breakpoint (wavefront_bialign.c:411-519)
+DI int find_breakpoint(Ctx& c, Dir& f, Dir& r, BP& bp) {
+  ring_init(c, f);
+  ring_init(c, r);
+  const int max_antidiagonal = f.plen + f.tlen - 1;
+  int sf = 0, sr = 0, f_max = 0, r_max = 0, ak = 0;
+  bp.score = INT_MAX;
+  c.guard = score_guard(c, f.plen, f.tlen);
+  if (extend_step(c, f, sf, &f_max)) return f.status;
+  if (extend_step(c, r, sr, &r_max)) return r.status;
+  bool last_forward = false;
+  for (;;) {
+    if (f_max + r_max >= max_antidiagonal) break;
+    if (sf + sr > c.guard) return kStGuard;
+    ++sf;
+    compute_step(c, f, sf);
+    if (c.spill) return kStCapacity;
+    const int fq = extend_step(c, f, sf, &ak);
+    if (f_max < ak) f_max = ak;
+    last_forward = true;
+    if (fq) return f.status;
+    if (f_max + r_max >= max_antidiagonal) break;
+    ++sr;
+    compute_step(c, r, sr);
+    if (c.spill) return kStCapacity;
+    const int rq = extend_step(c, r, sr, &ak);
+    if (r_max < ak) r_max = ak;
+    last_forward = false;
+    if (rq) return r.status;
+    if (sr + sf >= c.max_steps) return kStMaxSteps;
+  }
+  for (;;) {
+    if (last_forward) {
+      const int min_r = (sr > c.scope - 1) ? sr - (c.scope - 1) : 0;
+      if (sf + min_r - c.o >= bp.score) break;
+      overlap(c, f, r, sf, sr, true, bp);
+      ++sr;
+      compute_step(c, r, sr);
+      if (c.spill) return kStCapacity;
+      if (extend_step(c, r, sr, &ak)) return r.status;
+    }
+    const int min_f = (sf > c.scope - 1) ? sf - (c.scope - 1) : 0;
+    if (min_f + sr - c.o >= bp.score) break;
+    if (sf + sr > c.guard) return kStGuard;
+    overlap(c, r, f, sr, sf, false, bp);
+    ++sf;
+    compute_step(c, f, sf);
+    if (c.spill) return kStCapacity;
+    if (extend_step(c, f, sf, &ak)) return f.status;
+    if (sr + sf >= c.max_steps) return kStMaxSteps;
+    last_forward = true;
+  }
+  return kStOk;
+}
+
+// ---------------------------------------------------------------------------
+// output runs
+
+DI void out_flush(Ctx& c) {
+  if (c.pend_len <= 0) return;
+  if (c.out_n >= c.out_cap) {
+    c.cap_err = 1;
+  } else {
+    if (c.lane == 0) c.out[c.out_n] = ((uint32_t)c.pend_len << 4) | (uint32_t)c.pend_op;
+    c.out_n++;
+  }
+  // cigar_score_gap_affine (cigar.c:277-303) over maximal runs
+  switch (c.pend_op) {
+    case kOpM: c.alg_score -= c.um * c.pend_len; break;
+    case kOpX: c.alg_score -= c.ux * c.pend_len; break;
+    default: c.alg_score -= c.uo + c.ue * c.pend_len; break;
+  }
+  c.pend_len = 0;
+  c.pend_op = -1;
+}
+DI void out_append(Ctx& c, int op, int n) {
+  if (n <= 0) return;
+  if (op == c.pend_op) { c.pend_len += n; return; }
+  out_flush(c);
+  c.pend_op = op;
+  c.pend_len = n;
+}
+
+// reversed runs of one backtrace, merged as they are emitted
+struct RevRuns {
+  int n, op, len;
+};
+DI void rev_emit(Ctx& c, RevRuns& rr, int op, int n) {
+  if (n <= 0) return;
+  if (op == rr.op) { rr.len += n; return; }
+  if (rr.len > 0) {
+    if (rr.n >= c.L.rcap) { c.spill = 1; return; }
+    if (c.lane == 0) c.ws[c.L.runs_tmp + rr.n] = (int)(((uint32_t)rr.len << 4) | (uint32_t)rr.op);
+    rr.n++;
+  }
+  rr.op = op;
+  rr.len = n;
+}
+
+// ---------------------------------------------------------------------------
+// base case: wavefront_bialign_base (wavefront_bialign.c:155-185)
+
+constexpr int kBtM = 9, kBtD1Ext = 6, kBtD1Open = 5, kBtI1Ext = 2, kBtI1Open = 1;
+
+DI long long bt_src(const Ctx& c, int score, int comp, int k, int add, int type) {
+  if (score < 0) return (long long)kOffsetNull;
+  const int h = base_hdr(c, score, comp);
+  const int kb = uni(c.ws[h + 2]);
+  if (kb == kAbsent) return (long long)kOffsetNull;
+  const int lo = uni(c.ws[h]), hi = uni(c.ws[h + 1]);
+  if (!(lo <= k && k <= hi)) return (long long)kOffsetNull;
+  return ((long long)(uni(c.ws[kb + k]) + add) << 4) | type;
+}
+DI long long max64(long long a, long long b) { return a >= b ? a : b; }
+
+// wavefront_backtrace_affine (wavefront_backtrace.c:320-529); appends the
+// operations to the output in forward order.  Returns 0 or -1 on a trace error.
+DI int backtrace(Ctx& c, const Dir& b, int score) {
+  int matrix = b.ce;
+  int k = b.tlen - b.plen, off = b.tlen;
+  int h = off, v = off - k;
+  RevRuns rr{0, -1, 0};
+  if (matrix == CM) {
+    if (v < b.plen) rev_emit(c, rr, kOpD, b.plen - v);
+    if (h < b.tlen) rev_emit(c, rr, kOpI, b.tlen - h);
+  }
+  while (v > 0 && h > 0 && score > 0) {
+    const int mismatch = score - c.x;
+    const int gap_open1 = score - c.o - c.e;
+    const int gap_extend1 = score - c.e;
+    long long max_all;
+    if (matrix == CM) {
+      const long long misms = bt_src(c, mismatch, CM, k, 1, kBtM);
+      const long long io = bt_src(c, gap_open1, CM, k - 1, 1, kBtI1Open);
+      const long long ix = bt_src(c, gap_extend1, CI, k - 1, 1, kBtI1Ext);
+      const long long dop = bt_src(c, gap_open1, CM, k + 1, 0, kBtD1Open);
+      const long long dx = bt_src(c, gap_extend1, CD, k + 1, 0, kBtD1Ext);
+      max_all = max64(misms, max64(max64(io, ix), max64(dop, dx)));
+    } else if (matrix == CI) {
+      max_all = max64(bt_src(c, gap_open1, CM, k - 1, 1, kBtI1Open),
+                      bt_src(c, gap_extend1, CI, k - 1, 1, kBtI1Ext));
+    } else {
+      max_all = max64(bt_src(c, gap_open1, CM, k + 1, 0, kBtD1Open),
+                      bt_src(c, gap_extend1, CD, k + 1, 0, kBtD1Ext));
+    }
+    if (max_all < 0) break;
+    if (matrix == CM) {
+      const int max_off = (int)(max_all >> 4);
+      rev_emit(c, rr, kOpM, off - max_off);
+      off = max_off;
+      v = off - k;
+      h = off;
+      if (v <= 0 || h <= 0) break;
+    }
+    switch ((int)(max_all & 15)) {
+      case kBtM: score = mismatch; matrix = CM; rev_emit(c, rr, kOpX, 1); --off; break;
+      case kBtI1Open: score = gap_open1; matrix = CM; rev_emit(c, rr, kOpI, 1); --k; --off; break;
+      case kBtI1Ext: score = gap_extend1; matrix = CI; rev_emit(c, rr, kOpI, 1); --k; --off; break;
+      case kBtD1Open: score = gap_open1; matrix = CM; rev_emit(c, rr, kOpD, 1); ++k; break;
+      case kBtD1Ext: score = gap_extend1; matrix = CD; rev_emit(c, rr, kOpD, 1); ++k; break;
+      default: return -1;
+    }
+    if (c.spill) return 0;
+    v = off - k;
+    h = off;
+  }
+  if (matrix == CM) {
+    if (v > 0 && h > 0) {
+      const int nm = imin(v, h);
+      rev_emit(c, rr, kOpM, nm);
+      v -= nm;
+      h -= nm;
+    }
+    rev_emit(c, rr, kOpD, v);
+    rev_emit(c, rr, kOpI, h);
+  } else if (v != 0 || h != 0 || (score != 0 && c.um == 0)) {
+    return -1;
+  }
+  if (c.spill) return 0;
+  wave_sync();
+  // forward order: the pending run first, then the stored runs backwards
+  out_append(c, rr.op, rr.len);
+  for (int i = rr.n - 1; i >= 0; --i) {
+    const uint32_t run = (uint32_t)uni(c.ws[c.L.runs_tmp + i]);
+    out_append(c, (int)(run & 15u), (int)(run >> 4));
+  }
+  return 0;
+}
+
+// Unidirectional WFA over the sub-problem with all wavefronts kept
+// (wavefront_unialign.c:242-273).  Returns kStOk or kStUnattainable.
+DI int base_case(Ctx& c, int pb, int pe, int tb, int te, int cb, int ce, int* score_out) {
+  Dir b{pb, pe - pb, tb, te - tb, 0, -1, cb, ce, 0, kStOk, 0};
+  // score 0: the begin component at k=0 (wavefront_aligner.c:311-383)
+  c.arena_top = 1;
+  set_hdr(c, b, 0, CM, 0, 0, c.L.data, cb == CM);
+  set_hdr(c, b, 0, CI, 0, 0, c.L.data, cb == CI);
+  set_hdr(c, b, 0, CD, 0, 0, c.L.data, cb == CD);
+  c.ws[c.L.data] = 0;
+  wave_sync();
+  int score = 0, ak;
+  const int guard = score_guard(c, b.plen, b.tlen);
+  for (;;) {
+    if (score > guard) return kStGuard;
+    if (extend_step(c, b, score, &ak)) {
+      if (b.status != kStEndReached) return kStUnattainable;
+      if (backtrace(c, b, score) != 0) return kStUnattainable;
+      *score_out = score;
+      return kStOk;
+    }
+    ++score;
+    compute_step(c, b, score);
+    if (c.spill) return kStCapacity;
+    if (score >= c.max_steps) return kStUnattainable;   // reported as unattainable by the base
+  }
+}
+
+// ---------------------------------------------------------------------------
+// recursion driver: wavefront_bialign_alignment (wavefront_bialign.c:581-658)
+
+struct Sub {
+  int pb, pe, tb, te, cb, ce, rem, lvl;
+};
+DI void push(Ctx& c, int& sp, const Sub& s) {
+  if (sp >= c.L.scap) { c.spill = 1; return; }
+  int* e = c.ws + c.L.stack + sp * 8;
+  e[0] = s.pb; e[1] = s.pe; e[2] = s.tb; e[3] = s.te;
+  e[4] = s.cb; e[5] = s.ce; e[6] = s.rem; e[7] = s.lvl;
+  ++sp;
+}
+DI Sub pop(Ctx& c, int& sp) {
+  --sp;
+  const int* e = c.ws + c.L.stack + sp * 8;
+  return Sub{uni(e[0]), uni(e[1]), uni(e[2]), uni(e[3]), uni(e[4]), uni(e[5]), uni(e[6]), uni(e[7])};
+}
+
+// Aligns the staged pair.  Returns the final WFA status (wavefront_bialign.c:722-729).
+DI int align_pair(Ctx& c, int plen, int tlen, int score_only, int* wf_score) {
+  *wf_score = INT_MIN;
+  if (score_only) {
+    // wavefront_bialign_compute_score (wavefront_bialign.c:662-699)
+    Dir f{0, plen, 0, tlen, 0, 0, CM, CM, 0, kStOk, 0};
+    Dir r{0, plen, 0, tlen, 1, 1, CM, CM, 0, kStOk, 0};
+    BP bp;
+    const int st = find_breakpoint(c, f, r, bp);
+    if (c.spill) return kStCapacity;
+    if (st != kStOk && st != kStEndReached)
+      return (st == kStMaxSteps || st == kStGuard) ? st : kStUnattainable;
+    int s = bp.score;
+    if (st == kStEndReached) s = (f.status == kStEndReached) ? f.status_score : r.status_score;
+    *wf_score = s;
+    c.alg_score = (c.um == 0) ? -s : ((-c.um) * (plen + tlen) - s) / 2;
+    return 0;
+  }
+  int sp = 0;
+  const bool min_length = imax(plen, tlen) <= kFallbackMinLength;
+  push(c, sp, Sub{0, plen, 0, tlen, CM, CM, min_length ? 0 : INT_MAX, 0});
+  wave_sync();
+  int status = kStOk;
+  int iters = 0;
+  const int max_iters = 4 * (plen + tlen) + 64;
+  while (sp > 0) {
+    if (++iters > max_iters) { status = kStGuard; break; }
+    const Sub s = pop(c, sp);
+    const int pl = s.pe - s.pb, tl = s.te - s.tb;
+    if (tl == 0) { out_append(c, kOpD, pl); continue; }
+    if (pl == 0) { out_append(c, kOpI, tl); continue; }
+    int bscore = 0, st;
+    if (s.rem <= kFallbackMinScore) {
+      st = base_case(c, s.pb, s.pe, s.tb, s.te, s.cb, s.ce, &bscore);
+      if (c.spill) return kStCapacity;
+      if (s.lvl == 0) *wf_score = bscore;
+      if (st != kStOk) { status = st; break; }
+      continue;
+    }
+    Dir f{s.pb, pl, s.tb, tl, 0, 0, s.cb, s.ce, 0, kStOk, 0};
+    Dir r{s.pb, pl, s.tb, tl, 1, 1, s.ce, s.cb, 0, kStOk, 0};
+    BP bp;
+    st = find_breakpoint(c, f, r, bp);
+    if (c.spill) return kStCapacity;
+    if (st != kStOk) {   // wavefront_bialign_find_breakpoint_exception (:520-547)
+      if (st == kStEndReached) {
+        const int reached = (f.status == kStEndReached) ? f.status_score : r.status_score;
+        if (reached <= kRecoveryMinScore) {
+          st = base_case(c, s.pb, s.pe, s.tb, s.te, s.cb, s.ce, &bscore);
+          if (c.spill) return kStCapacity;
+          if (s.lvl == 0) *wf_score = bscore;
+          if (st != kStOk) { status = st; break; }
+          continue;
+        }
+        status = kStEndUnreachable;
+        break;
+      }
+      status = st;
+      break;
+    }
+    if (s.lvl == 0) *wf_score = bp.score;
+    const int bh = bp.of, bv = bp.of - bp.kf;
+    push(c, sp, Sub{s.pb + bv, s.pe, s.tb + bh, s.te, bp.comp, s.ce, bp.sr, s.lvl + 1});
+    push(c, sp, Sub{s.pb, s.pb + bv, s.tb, s.tb + bh, s.cb, bp.comp, bp.sf, s.lvl + 1});
+    if (c.spill) return kStCapacity;
+    wave_sync();
+  }
+  out_flush(c);
+  if (status == kStOk) return 0;
+  if (status == kStMaxSteps || status == kStGuard) return status;
+  return kStUnattainable;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+
+template <bool kLds>
+__global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
+  extern __shared__ int32_t lds[];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  int32_t* ws = kLds ? lds + wv * p.L.total
+                     : p.gws + (int64_t)(blockIdx.x * kWavesPerBlock + wv) * p.gws_stride;
+  const int count = p.work_count ? uni(*(volatile const int32_t*)p.work_count) : p.num_work;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(p.queue, p.chunk);
+    base = __shfl(base, 0, kWave);
+    base = uni(base);
+    if (base >= count) break;
+    const int end = imin(base + p.chunk, count);
+    for (int w = base; w < end; ++w) {
+      const int idx = p.work_list ? uni(p.work_list[w]) : w;
+      const int pair = p.order ? uni(p.order[idx]) : idx;
+      const int plen = uni(p.p_len[pair]), tlen = uni(p.t_len[pair]);
+      const uint64_t* gP = p.words + p.p_woff[pair];
+      const uint64_t* gT = p.words + p.t_woff[pair];
+      Ctx c;
+      c.ws = ws;
+      c.lane = lane;
+      c.L = p.L;
+      c.x = p.x; c.o = p.o; c.e = p.e; c.scope = p.scope;
+      c.max_steps = p.max_steps;
+      c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
+      c.arena_top = 0;
+      c.out = p.runs + p.runs_off[pair];
+      c.out_cap = (int)(p.runs_off[pair + 1] - p.runs_off[pair]);
+      c.out_n = 0; c.pend_op = -1; c.pend_len = 0; c.alg_score = 0;
+      c.spill = 0; c.cap_err = 0;
+      if (kLds) {
+        // stage both sequences: [pad][words...][pad][pad] (the pads keep the
+        // 64-bit windows in bounds at either end)
+        const int pw = (plen + 31) >> 5, tw = (tlen + 31) >> 5;
+        if (pw + 3 > p.L.seq_words || tw + 3 > p.L.seq_words) {
+          c.spill = 1;
+        } else {
+          uint64_t* sP = reinterpret_cast<uint64_t*>(ws + p.L.seq);
+          uint64_t* sT = sP + p.L.seq_words;
+          for (int i = lane; i < pw + 3; i += kWave) {
+            sP[i] = (i >= 1 && i <= pw) ? gP[i - 1] : 0ull;
+          }
+          for (int i = lane; i < tw + 3; i += kWave) {
+            sT[i] = (i >= 1 && i <= tw) ? gT[i - 1] : 0ull;
+          }
+          c.P = sP + 1;
+          c.T = sT + 1;
+          wave_sync();
+        }
+      } else {
+        c.P = gP;
+        c.T = gT;
+      }
+      int wf = INT_MIN, st = kStCapacity;
+      if (!c.spill) st = align_pair(c, plen, tlen, p.score_only, &wf);
+      if (c.spill) {
+        if (kLds) {
+          if (lane == 0) {
+            const int slot = atomicAdd(p.spill_count, 1);
+            p.spill_list[slot] = pair;
+          }
+          continue;
+        }
+        st = kStCapacity;
+      }
+      if (c.cap_err) st = kStCapacity;
+      if (lane == 0) {
+        p.status[pair] = st;
+        p.wf_score[pair] = wf;
+        p.score[pair] = (p.score_only || c.out_n > 0) ? c.alg_score : -1;
+        p.nruns[pair] = p.score_only ? 0 : c.out_n;
+      }
+    }
+  }
+}
+
+// Resets the queue/spill counters (a kernel rather than a memset so that the
+// launch sequence can be captured in a graph).
+__global__ void biwfa_reset(int32_t* counters, int n) {
+  const int i = threadIdx.x;
+  if (i < n) counters[i] = 0;
+}
+
+// Copies each pair's runs into the compacted layout (one wave per pair).
+__global__ __launch_bounds__(256) void biwfa_compact(const uint32_t* runs, const int64_t* runs_off,
+                                                     const int32_t* nruns, const int64_t* out_off,
+                                                     uint32_t* out, int n) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (wave >= n) return;
+  const uint32_t* src = runs + runs_off[wave];
+  uint32_t* dst = out + out_off[wave];
+  const int cnt = nruns[wave];
+  for (int i = lane; i < cnt; i += kWave) dst[i] = src[i];
+}
+
+template __global__ void biwfa_persistent<true>(KParams);
+template __global__ void biwfa_persistent<false>(KParams);
+
+}  // namespace bw

@@ -1,0 +1,719 @@
+// runtime.cpp -- host side of libbiwfa_mi355x: the C ABI declared in
+// include/biwfa_mi355x.h on top of the gfx950 kernels in biwfa_kernel.hip.
+//
+// One aligner = one device + one HIP stream + reusable device buffers.  A batch
+// is aligned by two launches of the persistent kernel: the LDS instantiation
+// over every pair in schedule order, then the global-workspace instantiation
+// over the pairs the first one spilled (its work count is read on the device,
+// so no host round trip sits between the launches).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "biwfa_device.hpp"
+#include "biwfa_mi355x.h"
+
+namespace bw {
+template <bool kLds>
+__global__ void biwfa_persistent(KParams p);
+__global__ void biwfa_reset(int32_t* counters, int n);
+__global__ void biwfa_compact(const uint32_t* runs, const int64_t* runs_off, const int32_t* nruns,
+                              const int64_t* out_off, uint32_t* out, int n);
+}  // namespace bw
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                   \
+      return BW_STATUS_HIP_ERROR;                                                      \
+    }                                                                                  \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count);
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Per-wave workspace layout (int32 units).
+bw::WsLayout make_layout(int scope, int wcap, int arena_min, int hcap, int rcap, int scap,
+                         int seq_words) {
+  bw::WsLayout L{};
+  int off = 0;
+  L.ring_hdr = off;
+  off += 2 * scope * 3 * 4;
+  L.data = off;
+  L.wcap = wcap;
+  const int ring_ints = 2 * scope * 3 * wcap;
+  L.arena_cap = std::max(ring_ints, arena_min);
+  off += L.arena_cap;
+  L.bhdr = off;
+  L.hcap = hcap;
+  off += hcap * 3 * 3;
+  L.runs_tmp = off;
+  L.rcap = rcap;
+  off += rcap;
+  L.stack = off;
+  L.scap = scap;
+  off += scap * 8;
+  off = (off + 3) & ~3;
+  L.seq = off;
+  L.seq_words = seq_words;
+  off += seq_words > 0 ? 2 * seq_words * 2 : 0;
+  L.total = (off + 3) & ~3;
+  return L;
+}
+
+struct Penalties {
+  int x, o, e, scope;     // internal
+  int um, ux, uo, ue;     // user
+};
+
+int internal_penalties(const bw_affine_penalties_t& p, Penalties* out) {
+  // wavefront_penalties_set_affine (wavefront_penalties.c:95-131)
+  if (p.match > 0 || p.mismatch <= 0 || p.gap_opening < 0 || p.gap_extension <= 0) {
+    set_error("penalties must satisfy match<=0, mismatch>0, gap_opening>=0, gap_extension>0");
+    return BW_STATUS_INVALID_ARG;
+  }
+  Penalties q;
+  q.um = p.match; q.ux = p.mismatch; q.uo = p.gap_opening; q.ue = p.gap_extension;
+  if (p.match < 0) {
+    q.x = 2 * p.mismatch - 2 * p.match;
+    q.o = 2 * p.gap_opening;
+    q.e = 2 * p.gap_extension - p.match;
+  } else {
+    q.x = p.mismatch; q.o = p.gap_opening; q.e = p.gap_extension;
+  }
+  // wavefront_components_dimensions_affine (wavefront_components.c:81-100)
+  q.scope = std::max(q.o + q.e, q.x) + 1;
+  if (q.scope > bw::kMaxScope) {
+    set_error("max(o+e, x)+1 exceeds the supported ring depth (32)");
+    return BW_STATUS_INVALID_ARG;
+  }
+  *out = q;
+  return 0;
+}
+
+}  // namespace
+
+struct bw_dbatch {
+  int64_t n = 0, nwords = 0;
+  uint64_t* d_words = nullptr;
+  int64_t* d_pwoff = nullptr;
+  int32_t* d_plen = nullptr;
+  int64_t* d_twoff = nullptr;
+  int32_t* d_tlen = nullptr;
+  std::vector<int32_t> h_plen, h_tlen;
+  int max_p = 0, max_t = 0, max_pt = 0;
+  int32_t* d_order = nullptr;
+  int32_t* d_spill = nullptr;
+  int32_t* d_score = nullptr;
+  int32_t* d_wf = nullptr;
+  int32_t* d_status = nullptr;
+  int32_t* d_nruns = nullptr;
+  uint32_t* d_runs = nullptr;
+  int64_t* d_runs_off = nullptr;
+  int64_t runs_cap = 0;
+  int32_t* d_gws = nullptr;
+  int64_t gws_bytes = 0;
+};
+
+struct bw_aligner {
+  bw_attr_t attr{};
+  Penalties pen{};
+  int dev = 0, cu_count = 0;
+  size_t lds_per_block_max = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  int32_t* d_counters = nullptr;   // [0] LDS queue, [1] spill count, [2] global queue
+  float last_total_ms = 0.f, last_main_ms = 0.f;
+  // last single-pair result
+  int status = 0, score = 0, wf_score = 0;
+  std::vector<uint32_t> runs;
+  std::vector<char> ops;
+  std::vector<uint32_t> sam;
+  // per-aligner schedule
+  int schedule = BW_SCHEDULE_LENGTH_SORTED;
+};
+
+extern "C" {
+
+const char* bw_last_error(void) { return g_last_error.c_str(); }
+
+const char* bw_strerror(int status) {
+  switch (status) {
+    case BW_STATUS_ALG_COMPLETED: return "[WFA] Alignment completed successfully";
+    case BW_STATUS_ALG_PARTIAL: return "[WFA] Alignment extension computed (partial alignment)";
+    case BW_STATUS_MAX_STEPS_REACHED: return "[WFA] Alignment failed. Maximum WFA-steps limit reached";
+    case BW_STATUS_OOM: return "[WFA] Alignment failed. Maximum memory limit reached";
+    case BW_STATUS_UNATTAINABLE: return "[WFA] Alignment failed. Unattainable under configured heuristics";
+    case BW_STATUS_DEVICE_CAPACITY: return "[BiWFA-MI355X] A per-pair device capacity was exceeded";
+    case BW_STATUS_HIP_ERROR: return "[BiWFA-MI355X] HIP runtime error";
+    case BW_STATUS_INVALID_ARG: return "[BiWFA-MI355X] Invalid argument";
+    case BW_STATUS_NO_DEVICE: return "[BiWFA-MI355X] No usable gfx950 device";
+    case BW_STATUS_BAD_SEQUENCE: return "[BiWFA-MI355X] Sequence byte outside ACGT";
+    case BW_STATUS_INTERNAL: return "[BiWFA-MI355X] Device termination guard tripped";
+    default: return "[WFA] Unknown error code";
+  }
+}
+
+void bw_attr_default(bw_attr_t* attr) {
+  std::memset(attr, 0, sizeof(*attr));
+  attr->penalties = bw_affine_penalties_t{0, 4, 6, 2};
+  attr->scope = BW_SCOPE_ALIGNMENT;
+  attr->max_alignment_steps = 0;
+  attr->device = 0;
+  attr->schedule = BW_SCHEDULE_LENGTH_SORTED;
+  attr->waves_per_cu = 0;
+  attr->queue_chunk = 0;
+}
+
+bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
+  bw_attr_t attr;
+  if (attr_in) attr = *attr_in; else bw_attr_default(&attr);
+  Penalties pen;
+  if (internal_penalties(attr.penalties, &pen) != 0) return nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    set_error("no HIP device available");
+    return nullptr;
+  }
+  if (attr.device < 0 || attr.device >= ndev) {
+    set_error("device ordinal out of range");
+    return nullptr;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, attr.device) != hipSuccess) {
+    set_error("hipGetDeviceProperties failed");
+    return nullptr;
+  }
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+    return nullptr;
+  }
+  bw_aligner* a = new bw_aligner();
+  a->attr = attr;
+  a->pen = pen;
+  a->dev = attr.device;
+  a->cu_count = prop.multiProcessorCount;
+  a->lds_per_block_max = prop.sharedMemPerBlock;
+  a->schedule = attr.schedule;
+  if (hipSetDevice(a->dev) != hipSuccess || hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||
+      dalloc(&a->d_counters, 4) != hipSuccess) {
+    set_error("stream / counter allocation failed");
+    delete a;
+    return nullptr;
+  }
+  for (auto& e : a->ev) (void)hipEventCreate(&e);
+  return a;
+}
+
+void bw_aligner_delete(bw_aligner_t* a) {
+  if (!a) return;
+  (void)hipSetDevice(a->dev);
+  if (a->stream) (void)hipStreamSynchronize(a->stream);
+  dfree(a->d_counters);
+  for (auto& e : a->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (a->stream) (void)hipStreamDestroy(a->stream);
+  delete a;
+}
+
+void bw_dbatch_free(bw_dbatch_t* b) {
+  if (!b) return;
+  dfree(b->d_words); dfree(b->d_pwoff); dfree(b->d_plen); dfree(b->d_twoff); dfree(b->d_tlen);
+  dfree(b->d_order); dfree(b->d_spill); dfree(b->d_score); dfree(b->d_wf); dfree(b->d_status);
+  dfree(b->d_nruns); dfree(b->d_runs); dfree(b->d_runs_off); dfree(b->d_gws);
+  delete b;
+}
+
+static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out) {
+  *out = nullptr;
+  if (!a || !in || in->num_pairs < 0 || (in->num_pairs > 0 && (!in->words || !in->pattern_length ||
+      !in->text_length || !in->pattern_word_offset || !in->text_word_offset))) {
+    set_error("invalid batch");
+    return BW_STATUS_INVALID_ARG;
+  }
+  if (in->num_pairs > INT32_MAX / 2) {
+    set_error("too many pairs in one batch");
+    return BW_STATUS_INVALID_ARG;
+  }
+  HIP_TRY(hipSetDevice(a->dev));
+  bw_dbatch* b = new bw_dbatch();
+  const int64_t n = in->num_pairs;
+  b->n = n;
+  b->nwords = in->num_words;
+  b->h_plen.assign(in->pattern_length, in->pattern_length + n);
+  b->h_tlen.assign(in->text_length, in->text_length + n);
+  std::vector<int64_t> runs_off(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int p = b->h_plen[i], t = b->h_tlen[i];
+    const int64_t pw = in->pattern_word_offset[i], tw = in->text_word_offset[i];
+    if (p < 0 || t < 0 || pw < 1 || tw < 1 || pw + (p + 31) / 32 + 2 > in->num_words ||
+        tw + (t + 31) / 32 + 2 > in->num_words) {
+      set_error("pair " + std::to_string(i) + ": bad length or word offset (need one pad word "
+                "before and two after every sequence)");
+      bw_dbatch_free(b);
+      return BW_STATUS_INVALID_ARG;
+    }
+    b->max_p = std::max(b->max_p, p);
+    b->max_t = std::max(b->max_t, t);
+    b->max_pt = std::max(b->max_pt, p + t);
+    runs_off[i + 1] = runs_off[i] + (int64_t)p + t + 1;
+  }
+  b->runs_cap = runs_off[n];
+  auto fail = [&](hipError_t e) {
+    set_error(std::string("device allocation/copy failed: ") + hipGetErrorString(e));
+    bw_dbatch_free(b);
+    return BW_STATUS_HIP_ERROR;
+  };
+  hipError_t e;
+  if ((e = dalloc(&b->d_words, in->num_words)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_pwoff, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_plen, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_twoff, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_tlen, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_order, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_spill, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_score, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_wf, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_status, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_nruns, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_runs, b->runs_cap)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_runs_off, n + 1)) != hipSuccess) return fail(e);
+  hipStream_t s = a->stream;
+  if ((e = hipMemcpyAsync(b->d_words, in->words, sizeof(uint64_t) * in->num_words, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+  if (n > 0) {
+    if ((e = hipMemcpyAsync(b->d_pwoff, in->pattern_word_offset, sizeof(int64_t) * n, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(b->d_plen, in->pattern_length, sizeof(int32_t) * n, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(b->d_twoff, in->text_word_offset, sizeof(int64_t) * n, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(b->d_tlen, in->text_length, sizeof(int32_t) * n, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+  }
+  if ((e = hipMemcpyAsync(b->d_runs_off, runs_off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e);
+  *out = b;
+  const int st = bw_set_schedule(a, b, a->schedule, nullptr);
+  if (st != 0) {
+    bw_dbatch_free(b);
+    *out = nullptr;
+    return st;
+  }
+  return 0;
+}
+
+bw_dbatch_t* bw_dbatch_upload(bw_aligner_t* a, const bw_batch_t* in) {
+  bw_dbatch_t* b = nullptr;
+  upload_impl(a, in, &b);
+  return b;
+}
+
+int bw_set_schedule(bw_aligner_t* a, bw_dbatch_t* b, int schedule, const float* cost) {
+  if (!a || !b) return BW_STATUS_INVALID_ARG;
+  if (schedule == BW_SCHEDULE_LEARNED && !cost) {
+    set_error("learned schedule needs one predicted cost per pair");
+    return BW_STATUS_INVALID_ARG;
+  }
+  const int64_t n = b->n;
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  if (schedule == BW_SCHEDULE_LENGTH_SORTED) {
+    std::stable_sort(order.begin(), order.end(), [&](int32_t i, int32_t j) {
+      return (b->h_plen[i] + b->h_tlen[i]) > (b->h_plen[j] + b->h_tlen[j]);
+    });
+  } else if (schedule == BW_SCHEDULE_LEARNED) {
+    std::stable_sort(order.begin(), order.end(), [&](int32_t i, int32_t j) { return cost[i] > cost[j]; });
+  } else if (schedule != BW_SCHEDULE_INPUT_ORDER) {
+    set_error("unknown schedule");
+    return BW_STATUS_INVALID_ARG;
+  }
+  a->schedule = schedule;
+  HIP_TRY(hipSetDevice(a->dev));
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(b->d_order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, a->stream));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+  }
+  return 0;
+}
+
+// Launch plan for one batch.
+struct Plan {
+  bool use_lds = false;
+  bw::WsLayout L_lds{}, L_glb{};
+  int lds_waves_per_block = 4, lds_blocks = 0;
+  int glb_blocks = 0;
+  int64_t glb_stride = 0;
+  int chunk = 4;
+};
+
+static Plan make_plan(const bw_aligner_t* a, const bw_dbatch_t* b) {
+  Plan pl;
+  const Penalties& q = a->pen;
+  const int pt = b->max_pt;
+  const int scope = q.scope;
+  // --- LDS instantiation: short pairs (ring width 32/64 diagonals) ---
+  const int maxlen = std::max(b->max_p, b->max_t);
+  if (maxlen <= 2048) {
+    const int wcap = (maxlen <= 400) ? 32 : 64;
+    const int seq_words = (maxlen + 31) / 32 + 3;
+    pl.L_lds = make_layout(scope, wcap, 0, 64, 128, 16, seq_words);
+    const size_t bytes_per_wave = (size_t)pl.L_lds.total * 4;
+    int wpb = 4;
+    while (wpb > 1 && bytes_per_wave * wpb > 64 * 1024) wpb >>= 1;
+    if (bytes_per_wave * wpb <= a->lds_per_block_max) {
+      pl.use_lds = true;
+      pl.lds_waves_per_block = wpb;
+      int blocks_per_cu = (int)((160 * 1024) / (bytes_per_wave * wpb));
+      blocks_per_cu = std::max(1, std::min(blocks_per_cu, 32 / wpb));
+      if (a->attr.waves_per_cu > 0) blocks_per_cu = std::max(1, a->attr.waves_per_cu / wpb);
+      pl.lds_blocks = a->cu_count * blocks_per_cu;
+    }
+  }
+  // --- global instantiation: capacities from the batch maxima ---
+  const int wcap = next_pow2(pt + 8);
+  const int min_length_bound = 100 * q.x + q.o + 100 * q.e;  // max(p,t) <= 100
+  int smax = std::max(bw::kRecoveryMinScore, bw::kFallbackMinScore) + 1;
+  if (std::max(b->max_p, b->max_t) <= bw::kFallbackMinLength) smax = std::max(smax, min_length_bound + 1);
+  else smax = std::max(smax, min_length_bound + 1);
+  const int width = std::min(2 * smax / q.e + 3, pt + 3);
+  const int64_t arena = std::min<int64_t>((int64_t)3 * (smax + 1) * width, (int64_t)1 << 28);
+  const int steps = std::min(pt, smax / std::max(1, std::min(q.x, q.e)) + 1);
+  const int rcap = 2 * steps + 16;
+  pl.L_glb = make_layout(scope, wcap, (int)arena, smax + 2, rcap, 64, 0);
+  pl.glb_stride = pl.L_glb.total;
+  const int64_t bytes_per_wave = pl.glb_stride * 4;
+  const int64_t budget = (int64_t)16 << 30;
+  int64_t waves = std::max<int64_t>(1, std::min<int64_t>(budget / bytes_per_wave, (int64_t)a->cu_count * 8));
+  if (!pl.use_lds) waves = std::min<int64_t>(waves, std::max<int64_t>(1, b->n));
+  pl.glb_blocks = (int)std::max<int64_t>(1, waves / bw::kWavesPerBlock);
+  pl.chunk = a->attr.queue_chunk > 0 ? a->attr.queue_chunk : (pl.use_lds ? 4 : 1);
+  return pl;
+}
+
+static bw::KParams base_params(const bw_aligner_t* a, const bw_dbatch_t* b) {
+  bw::KParams p{};
+  p.words = b->d_words;
+  p.p_woff = b->d_pwoff; p.p_len = b->d_plen;
+  p.t_woff = b->d_twoff; p.t_len = b->d_tlen;
+  p.x = a->pen.x; p.o = a->pen.o; p.e = a->pen.e; p.scope = a->pen.scope;
+  p.um = a->pen.um; p.ux = a->pen.ux; p.uo = a->pen.uo; p.ue = a->pen.ue;
+  p.score_only = a->attr.scope == BW_SCOPE_SCORE;
+  p.max_steps = a->attr.max_alignment_steps > 0 ? a->attr.max_alignment_steps : INT32_MAX;
+  p.score = b->d_score; p.wf_score = b->d_wf; p.status = b->d_status;
+  p.runs = b->d_runs; p.runs_off = b->d_runs_off; p.nruns = b->d_nruns;
+  return p;
+}
+
+int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
+  if (!a || !b) return BW_STATUS_INVALID_ARG;
+  HIP_TRY(hipSetDevice(a->dev));
+  if (b->n == 0) return 0;
+  const Plan pl = make_plan(a, b);
+  // global workspace (kept with the batch, grown on demand)
+  const int64_t gbytes = (int64_t)pl.glb_blocks * bw::kWavesPerBlock * pl.glb_stride * 4;
+  if (gbytes > b->gws_bytes) {
+    dfree(b->d_gws);
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b->d_gws), gbytes));
+    b->gws_bytes = gbytes;
+  }
+  hipStream_t s = a->stream;
+  HIP_TRY(hipEventRecord(a->ev[0], s));
+  hipLaunchKernelGGL(bw::biwfa_reset, dim3(1), dim3(64), 0, s, a->d_counters, 4);
+  bw::KParams p = base_params(a, b);
+  p.gws = b->d_gws;
+  p.gws_stride = pl.glb_stride;
+  if (pl.use_lds) {
+    p.order = b->d_order;
+    p.work_list = nullptr;
+    p.work_count = nullptr;
+    p.num_work = (int32_t)b->n;
+    p.queue = a->d_counters + 0;
+    p.chunk = pl.chunk;
+    p.spill_count = a->d_counters + 1;
+    p.spill_list = b->d_spill;
+    p.L = pl.L_lds;
+    const size_t lds = (size_t)pl.L_lds.total * 4 * pl.lds_waves_per_block;
+    HIP_TRY(hipEventRecord(a->ev[1], s));
+    hipLaunchKernelGGL(bw::biwfa_persistent<true>, dim3(pl.lds_blocks),
+                       dim3(bw::kWave * pl.lds_waves_per_block), lds, s, p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(a->ev[2], s));
+    // spilled pairs
+    p.order = nullptr;
+    p.work_list = b->d_spill;
+    p.work_count = a->d_counters + 1;
+    p.num_work = 0;
+  } else {
+    p.order = b->d_order;
+    p.work_list = nullptr;
+    p.work_count = nullptr;
+    p.num_work = (int32_t)b->n;
+    HIP_TRY(hipEventRecord(a->ev[1], s));
+  }
+  p.queue = a->d_counters + 2;
+  p.chunk = 1;
+  p.spill_count = nullptr;
+  p.spill_list = nullptr;
+  p.L = pl.L_glb;
+  hipLaunchKernelGGL(bw::biwfa_persistent<false>, dim3(pl.glb_blocks), dim3(bw::kWave * bw::kWavesPerBlock),
+                     0, s, p);
+  HIP_TRY(hipGetLastError());
+  if (!pl.use_lds) HIP_TRY(hipEventRecord(a->ev[2], s));
+  return 0;
+}
+
+int bw_synchronize(bw_aligner_t* a) {
+  if (!a) return BW_STATUS_INVALID_ARG;
+  HIP_TRY(hipSetDevice(a->dev));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  float total = 0.f, main_ms = 0.f;
+  if (hipEventElapsedTime(&total, a->ev[0], a->ev[2]) == hipSuccess) a->last_total_ms = total;
+  if (hipEventElapsedTime(&main_ms, a->ev[1], a->ev[2]) == hipSuccess) a->last_main_ms = main_ms;
+  return 0;
+}
+
+int bw_last_timing(const bw_aligner_t* a, float* total_ms, float* main_ms) {
+  if (!a) return BW_STATUS_INVALID_ARG;
+  if (total_ms) *total_ms = a->last_total_ms;
+  if (main_ms) *main_ms = a->last_main_ms;
+  return 0;
+}
+
+int bw_dbatch_download(bw_aligner_t* a, bw_dbatch_t* b, bw_result_t* r) {
+  if (!a || !b || !r) return BW_STATUS_INVALID_ARG;
+  HIP_TRY(hipSetDevice(a->dev));
+  const int64_t n = b->n;
+  hipStream_t s = a->stream;
+  HIP_TRY(hipStreamSynchronize(s));
+  if (n == 0) {
+    if (r->cigar_offset) r->cigar_offset[0] = 0;
+    r->cigar_total = 0;
+    return 0;
+  }
+  if (r->score) HIP_TRY(hipMemcpyAsync(r->score, b->d_score, 4 * n, hipMemcpyDeviceToHost, s));
+  if (r->wf_score) HIP_TRY(hipMemcpyAsync(r->wf_score, b->d_wf, 4 * n, hipMemcpyDeviceToHost, s));
+  if (r->status) HIP_TRY(hipMemcpyAsync(r->status, b->d_status, 4 * n, hipMemcpyDeviceToHost, s));
+  std::vector<int32_t> nruns(n);
+  HIP_TRY(hipMemcpyAsync(nruns.data(), b->d_nruns, 4 * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  std::vector<int64_t> off(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) off[i + 1] = off[i] + std::max(0, nruns[i]);
+  r->cigar_total = off[n];
+  if (r->cigar_offset) std::memcpy(r->cigar_offset, off.data(), sizeof(int64_t) * (n + 1));
+  if (r->cigar_runs) {
+    if (off[n] > r->cigar_capacity) {
+      set_error("cigar_capacity too small: need " + std::to_string(off[n]));
+      return BW_STATUS_INVALID_ARG;
+    }
+    if (off[n] > 0) {
+      int64_t* d_off = nullptr;
+      uint32_t* d_out = nullptr;
+      HIP_TRY(dalloc(&d_off, n + 1));
+      HIP_TRY(dalloc(&d_out, off[n]));
+      HIP_TRY(hipMemcpyAsync(d_off, off.data(), 8 * (n + 1), hipMemcpyHostToDevice, s));
+      const int threads = 256, waves_per_block = threads / bw::kWave;
+      const int blocks = (int)((n + waves_per_block - 1) / waves_per_block);
+      hipLaunchKernelGGL(bw::biwfa_compact, dim3(blocks), dim3(threads), 0, s, b->d_runs,
+                         b->d_runs_off, b->d_nruns, d_off, d_out, (int)n);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(r->cigar_runs, d_out, 4 * off[n], hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      dfree(d_off);
+      dfree(d_out);
+    }
+  }
+  return 0;
+}
+
+int bw_align_batch(bw_aligner_t* a, const bw_batch_t* batch, bw_result_t* result) {
+  bw_dbatch_t* b = nullptr;
+  int st = upload_impl(a, batch, &b);
+  if (st != 0) return st;
+  st = bw_dbatch_align(a, b);
+  if (st == 0) st = bw_synchronize(a);
+  if (st == 0) st = bw_dbatch_download(a, b, result);
+  bw_dbatch_free(b);
+  return st;
+}
+
+// ---------------- single pair (WFA2-style) ----------------
+
+static int align_one(bw_aligner_t* a, const std::vector<uint64_t>& words, int plen, int tlen) {
+  const int pw = (plen + 31) / 32, tw = (tlen + 31) / 32;
+  const int64_t pwoff = 1, twoff = 1 + pw;
+  bw_batch_t bt;
+  bt.num_pairs = 1;
+  bt.num_words = (int64_t)words.size();
+  bt.words = words.data();
+  bt.pattern_word_offset = &pwoff;
+  bt.pattern_length = &plen;
+  bt.text_word_offset = &twoff;
+  bt.text_length = &tlen;
+  (void)tw;
+  int32_t score = 0, wf = 0, status = 0;
+  int64_t off[2] = {0, 0};
+  a->runs.assign((size_t)plen + tlen + 1, 0u);
+  bw_result_t r{};
+  r.score = &score;
+  r.wf_score = &wf;
+  r.status = &status;
+  r.cigar_offset = off;
+  r.cigar_runs = a->runs.data();
+  r.cigar_capacity = (int64_t)a->runs.size();
+  const int st = bw_align_batch(a, &bt, &r);
+  if (st != 0) {
+    a->status = st;
+    a->runs.clear();
+    a->ops.clear();
+    return st;
+  }
+  a->runs.resize((size_t)off[1]);
+  a->status = status;
+  a->score = score;
+  a->wf_score = wf;
+  a->ops.clear();
+  for (uint32_t run : a->runs) {
+    const int op = run & 15u;
+    const char ch = op == BW_OP_MATCH ? 'M' : op == BW_OP_MISMATCH ? 'X' : op == BW_OP_INSERTION ? 'I' : 'D';
+    a->ops.insert(a->ops.end(), run >> 4, ch);
+  }
+  return status;
+}
+
+static int pack_ascii(const char* seq, int len, uint64_t* w) {
+  for (int b = 0; b < len; b += 32) {
+    uint64_t acc = 0;
+    for (int j = 0; j < 32 && b + j < len; ++j) {
+      uint64_t code;
+      switch (seq[b + j]) {
+        case 'A': case 'a': code = 0; break;
+        case 'C': case 'c': code = 1; break;
+        case 'G': case 'g': code = 2; break;
+        case 'T': case 't': code = 3; break;
+        default: return -1;
+      }
+      acc |= code << (2 * j);
+    }
+    w[b / 32] = acc;
+  }
+  return 0;
+}
+
+int bw_align(bw_aligner_t* a, const char* pattern, int plen, const char* text, int tlen) {
+  if (!a || plen < 0 || tlen < 0 || (plen > 0 && !pattern) || (tlen > 0 && !text)) return BW_STATUS_INVALID_ARG;
+  const int pw = (plen + 31) / 32, tw = (tlen + 31) / 32;
+  std::vector<uint64_t> words((size_t)1 + pw + tw + 2, 0ull);
+  if (pack_ascii(pattern, plen, words.data() + 1) != 0 || pack_ascii(text, tlen, words.data() + 1 + pw) != 0) {
+    set_error("sequence byte outside ACGT");
+    a->status = BW_STATUS_BAD_SEQUENCE;
+    return BW_STATUS_BAD_SEQUENCE;
+  }
+  return align_one(a, words, plen, tlen);
+}
+
+int bw_align_packed2bits(bw_aligner_t* a, const uint8_t* pattern, int plen, const uint8_t* text, int tlen) {
+  if (!a || plen < 0 || tlen < 0 || (plen > 0 && !pattern) || (tlen > 0 && !text)) return BW_STATUS_INVALID_ARG;
+  // 4 bases per byte, LSB first: byte j of the input is bits 8*(j%8).. of word j/8.
+  const int pw = (plen + 31) / 32, tw = (tlen + 31) / 32;
+  std::vector<uint64_t> words((size_t)1 + pw + tw + 2, 0ull);
+  uint8_t* wp = reinterpret_cast<uint8_t*>(words.data() + 1);
+  uint8_t* wt = reinterpret_cast<uint8_t*>(words.data() + 1 + pw);
+  std::memcpy(wp, pattern, (size_t)(plen + 3) / 4);
+  std::memcpy(wt, text, (size_t)(tlen + 3) / 4);
+  return align_one(a, words, plen, tlen);
+}
+
+int bw_get_score(const bw_aligner_t* a) { return a ? a->score : 0; }
+int bw_get_wf_score(const bw_aligner_t* a) { return a ? a->wf_score : 0; }
+int bw_get_status(const bw_aligner_t* a) { return a ? a->status : BW_STATUS_INVALID_ARG; }
+
+int bw_get_cigar_ops(const bw_aligner_t* a, const char** ops, int* len) {
+  if (!a || !ops || !len) return BW_STATUS_INVALID_ARG;
+  *ops = a->ops.data();
+  *len = (int)a->ops.size();
+  return 0;
+}
+
+int bw_get_sam_cigar(bw_aligner_t* a, int show_mismatches, const uint32_t** buf, int* len) {
+  if (!a || !buf || !len) return BW_STATUS_INVALID_ARG;
+  if (show_mismatches) {
+    *buf = a->runs.data();
+    *len = (int)a->runs.size();
+    return 0;
+  }
+  // fold X into M (SAM op 0) and merge (cigar_compute_CIGAR, cigar.c:181-229)
+  a->sam.clear();
+  for (uint32_t run : a->runs) {
+    uint32_t op = run & 15u, n = run >> 4;
+    if (op == BW_OP_MATCH || op == BW_OP_MISMATCH) op = 0;
+    if (!a->sam.empty() && (a->sam.back() & 15u) == op) a->sam.back() += n << 4;
+    else a->sam.push_back((n << 4) | op);
+  }
+  *buf = a->sam.data();
+  *len = (int)a->sam.size();
+  return 0;
+}
+
+int bw_cigar_sprint(const bw_aligner_t* a, char* buffer, int cap) {
+  if (!a || !buffer || cap <= 0) return -1;
+  int cur = 0;
+  for (uint32_t run : a->runs) {
+    const int op = run & 15u;
+    const char ch = op == BW_OP_MATCH ? 'M' : op == BW_OP_MISMATCH ? 'X' : op == BW_OP_INSERTION ? 'I' : 'D';
+    const int w = std::snprintf(buffer + cur, (size_t)(cap - cur), "%u%c", run >> 4, ch);
+    if (w < 0 || cur + w >= cap) return -1;
+    cur += w;
+  }
+  buffer[cur] = '\0';
+  return cur;
+}
+
+int bw_align_pair(const char* pattern, int plen, const char* text, int tlen, int x, int o, int e,
+                  int32_t* score, char* ops, int ops_cap, int* ops_len) {
+  static thread_local bw_aligner_t* tl_aligner = nullptr;
+  static thread_local bw_affine_penalties_t tl_pen{0, 0, 0, 0};
+  if (!tl_aligner || tl_pen.mismatch != x || tl_pen.gap_opening != o || tl_pen.gap_extension != e) {
+    if (tl_aligner) bw_aligner_delete(tl_aligner);
+    bw_attr_t attr;
+    bw_attr_default(&attr);
+    attr.penalties = bw_affine_penalties_t{0, x, o, e};
+    tl_aligner = bw_aligner_new(&attr);
+    tl_pen = attr.penalties;
+    if (!tl_aligner) return BW_STATUS_NO_DEVICE;
+  }
+  const int st = bw_align(tl_aligner, pattern, plen, text, tlen);
+  if (score) *score = tl_aligner->score;
+  if (ops_len) *ops_len = (int)tl_aligner->ops.size();
+  if (ops) {
+    if ((int)tl_aligner->ops.size() > ops_cap) return BW_STATUS_INVALID_ARG;
+    std::memcpy(ops, tl_aligner->ops.data(), tl_aligner->ops.size());
+  }
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+/*
+ * biwfa_mi355x.h -- C ABI of libbiwfa_mi355x.so, the MI355X (gfx950) bidirectional
+ * wavefront aligner.  Plain C types only; every pointer is host memory unless
+ * the function says otherwise.
+ *
+ * Results are bit-exact with WFA2-lib v2.3 `wavefront_align` configured for
+ * exact BiWFA: gap_affine penalties, memory_mode = ultralow, heuristic = none
+ * (score from the CIGAR as align_benchmark writes it, CIGAR operations byte for
+ * byte).  Paths below are relative to the reference checkout.
+ *
+ * Three layers:
+ *   (i)   WFA2-style aligner object   -- bw_aligner_new / bw_align / bw_align_packed2bits /
+ *                                        bw_aligner_delete + result accessors;
+ *   (ii)  repo-style align-a-pair     -- bw_align_pair;
+ *   (iii) batch                       -- bw_align_batch (host buffers in/out) and the
+ *                                        device-resident bw_dbatch_* calls (inputs and
+ *                                        outputs stay in HBM; what bench.py times).
+ * Plus the seeded generator / 2-bit packer that feed the batch path.
+ */
+#ifndef BIWFA_MI355X_H_
+#define BIWFA_MI355X_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: WFA2-lib wavefront/wfa.h:46-55, plus device-side codes ---- */
+#define BW_STATUS_ALG_COMPLETED      0
+#define BW_STATUS_ALG_PARTIAL        1
+#define BW_STATUS_MAX_STEPS_REACHED  -100
+#define BW_STATUS_OOM                -200
+#define BW_STATUS_UNATTAINABLE       -300
+#define BW_STATUS_DEVICE_CAPACITY    -400  /* a per-pair device limit was exceeded      */
+#define BW_STATUS_HIP_ERROR          -401  /* a HIP runtime call failed                 */
+#define BW_STATUS_INVALID_ARG        -402  /* bad penalties / lengths / pointers        */
+#define BW_STATUS_NO_DEVICE          -403  /* no usable gfx950 device                   */
+#define BW_STATUS_BAD_SEQUENCE       -404  /* byte outside ACGT in a 2-bit packed path  */
+#define BW_STATUS_INTERNAL           -405  /* device termination guard tripped          */
+
+/* CIGAR runs use WFA2's SAM encoding with mismatches shown
+ * (cigar_get_CIGAR(show_mismatches=true), alignment/cigar.c:181-240):
+ * (length << 4) | op, op = 7 '=' (match, printed 'M'), 8 'X', 1 'I', 2 'D'. */
+#define BW_OP_MATCH 7
+#define BW_OP_MISMATCH 8
+#define BW_OP_INSERTION 1
+#define BW_OP_DELETION 2
+
+typedef struct {
+  int match;          /* <= 0 (match < 0 uses WFA2's Eizenga transform) */
+  int mismatch;       /* > 0  */
+  int gap_opening;    /* >= 0 */
+  int gap_extension;  /* > 0  */
+} bw_affine_penalties_t;                 /* alignment/affine_penalties.h */
+
+typedef enum {
+  BW_SCOPE_ALIGNMENT = 0,                /* score + CIGAR (compute_alignment) */
+  BW_SCOPE_SCORE = 1                     /* score only    (compute_score)     */
+} bw_scope_t;
+
+typedef enum {
+  BW_SCHEDULE_INPUT_ORDER = 0,           /* pairs taken in input order                     */
+  BW_SCHEDULE_LENGTH_SORTED = 1,         /* plain: longest (p+t) first                     */
+  BW_SCHEDULE_LEARNED = 2                /* by the cost model's predicted work, largest first */
+} bw_schedule_t;
+
+typedef struct {
+  bw_affine_penalties_t penalties;       /* default {0,4,6,2}                          */
+  int scope;                             /* bw_scope_t                                 */
+  int max_alignment_steps;               /* <= 0: unlimited (wavefront_attributes.c:89) */
+  int device;                            /* HIP device ordinal                         */
+  int schedule;                          /* bw_schedule_t                              */
+  int waves_per_cu;                      /* 0: autotuned                               */
+  int queue_chunk;                       /* 0: autotuned                               */
+} bw_attr_t;
+
+typedef struct bw_aligner bw_aligner_t;
+
+/* Fills *attr with the defaults above. */
+void bw_attr_default(bw_attr_t* attr);
+
+/* ---------------- (i) WFA2-style aligner ---------------- */
+
+/* Replaces wavefront_aligner_new (wavefront/wavefront_aligner.c:421-463).
+ * Returns NULL on invalid attributes or when no gfx950 device is usable
+ * (bw_last_error() tells which). */
+bw_aligner_t* bw_aligner_new(const bw_attr_t* attr);
+
+/* Replaces wavefront_aligner_delete (wavefront/wavefront_aligner.c:476-501). */
+void bw_aligner_delete(bw_aligner_t* aligner);
+
+/* Replaces wavefront_align (wavefront/wavefront_align.c:212-241).  Sequences are
+ * ASCII ACGT (either case).  Returns the status (BW_STATUS_*). */
+int bw_align(bw_aligner_t* aligner, const char* pattern, int pattern_length,
+             const char* text, int text_length);
+
+/* Replaces wavefront_align_packed2bits (wavefront/wavefront_align.c:182-211):
+ * 4 bases per byte, LSB first, A=0 C=1 G=2 T=3 (wavefront_sequences.c:102-140).
+ * All ceil(len/4) bytes are decoded. */
+int bw_align_packed2bits(bw_aligner_t* aligner, const uint8_t* pattern, int pattern_length,
+                         const uint8_t* text, int text_length);
+
+/* Score of the last alignment, as align_benchmark reports it
+ * (tools/align_benchmark/benchmark/benchmark_utils.c:143-179): the gap-affine
+ * score of the CIGAR (<= 0), or the score-only result in BW_SCOPE_SCORE. */
+int bw_get_score(const bw_aligner_t* aligner);
+
+/* Internal WFA score (penalty units, >= 0) of the last alignment. */
+int bw_get_wf_score(const bw_aligner_t* aligner);
+
+/* Status of the last alignment. */
+int bw_get_status(const bw_aligner_t* aligner);
+
+/* CIGAR operations of the last alignment, one char per column ('M','X','I','D'),
+ * like cigar->operations[begin_offset,end_offset).  Valid until the next call. */
+int bw_get_cigar_ops(const bw_aligner_t* aligner, const char** ops, int* length);
+
+/* SAM-packed CIGAR, cigar_get_CIGAR (alignment/cigar.c:230-240).  show_mismatches=1
+ * gives the run encoding above; 0 folds X into M (op 0).  Valid until the next call. */
+int bw_get_sam_cigar(bw_aligner_t* aligner, int show_mismatches,
+                     const uint32_t** cigar_buffer, int* cigar_length);
+
+/* Run-length text, cigar_sprint(print_matches=true) (alignment/cigar.c:706-740),
+ * e.g. "14M1I86M".  Returns the length written (excluding NUL) or -1 if cap is too small. */
+int bw_cigar_sprint(const bw_aligner_t* aligner, char* buffer, int capacity);
+
+/* Message for the last error on this thread. */
+const char* bw_last_error(void);
+
+/* Text of a status code (wavefront_align_strerror, wavefront_aligner.c:66-83). */
+const char* bw_strerror(int status);
+
+/* ---------------- (ii) repo-style align-a-pair ---------------- */
+
+/* The repo's per-pair entry (src/biWFA.cu:607 kernel + :756 driver, penalties
+ * x/o/e from src/headers/biWFA.h:10-12, match 0), extended to return the CIGAR.
+ * *score receives the (<= 0) score; ops receives 'M','X','I','D' (no NUL);
+ * *ops_length its length.  Uses a per-thread default aligner on device 0. */
+int bw_align_pair(const char* pattern, int pattern_length, const char* text, int text_length,
+                  int x, int o, int e, int32_t* score, char* ops, int ops_capacity,
+                  int* ops_length);
+
+/* ---------------- (iii) batch ---------------- */
+
+/* A batch of 2-bit packed pairs.  Each sequence starts on a 64-bit word boundary:
+ * base j of a sequence at bits 2*(j%32) of word (word_offset + j/32).  The word
+ * array needs one readable word before the first sequence and two after the last
+ * (bw_batch_words_needed accounts for them; the library never reads them as bases). */
+typedef struct {
+  int64_t num_pairs;
+  int64_t num_words;
+  const uint64_t* words;
+  const int64_t* pattern_word_offset;
+  const int32_t* pattern_length;
+  const int64_t* text_word_offset;
+  const int32_t* text_length;
+} bw_batch_t;
+
+/* Per-pair results.  cigar_offset[i]..cigar_offset[i+1] delimits pair i's runs
+ * in cigar_runs (compacted, input order).  Any pointer may be NULL to skip it. */
+typedef struct {
+  int32_t* score;           /* [num_pairs] .alg score              */
+  int32_t* wf_score;        /* [num_pairs] internal WFA score      */
+  int32_t* status;          /* [num_pairs]                         */
+  int64_t* cigar_offset;    /* [num_pairs + 1]                     */
+  uint32_t* cigar_runs;     /* [cigar_capacity]                    */
+  int64_t cigar_capacity;
+  int64_t cigar_total;      /* out: runs written                   */
+} bw_result_t;
+
+/* Aligns a host batch: upload, one persistent launch (plus a spill launch when
+ * needed), compaction, download.  Returns 0 or a BW_STATUS_* error. */
+int bw_align_batch(bw_aligner_t* aligner, const bw_batch_t* batch, bw_result_t* result);
+
+/* Device-resident batch: inputs uploaded once, results kept in HBM. */
+typedef struct bw_dbatch bw_dbatch_t;
+bw_dbatch_t* bw_dbatch_upload(bw_aligner_t* aligner, const bw_batch_t* batch);
+/* Enqueues the alignment of every pair on the aligner's stream (no host sync).
+ * Scheduling order comes from the aligner's schedule (set by bw_set_schedule). */
+int bw_dbatch_align(bw_aligner_t* aligner, bw_dbatch_t* dbatch);
+/* Waits for the aligner's stream. */
+int bw_synchronize(bw_aligner_t* aligner);
+/* Copies results to host buffers (compacting the runs). */
+int bw_dbatch_download(bw_aligner_t* aligner, bw_dbatch_t* dbatch, bw_result_t* result);
+void bw_dbatch_free(bw_dbatch_t* dbatch);
+
+/* Milliseconds of the last bw_dbatch_align, measured with HIP events on the
+ * aligner's stream; *main_ms is the persistent alignment kernel alone. */
+int bw_last_timing(const bw_aligner_t* aligner, float* total_ms, float* main_ms);
+
+/* ---------------- (iv) scheduling ---------------- */
+
+/* Selects the pair order of later bw_dbatch_align calls (bw_schedule_t).  For
+ * BW_SCHEDULE_LEARNED, `predicted_cost` holds one value per pair (host); larger
+ * values start first.  The order never changes any pair's result. */
+int bw_set_schedule(bw_aligner_t* aligner, bw_dbatch_t* dbatch, int schedule,
+                    const float* predicted_cost);
+
+/* ---------------- generator / packing ---------------- */
+
+typedef struct {
+  int64_t num_pairs;
+  uint64_t seed;
+  int32_t len_min, len_max;      /* pattern length; log-uniform when len_max > len_min */
+  int32_t num_edits;             /* >= 0: fixed number of edit events per pair          */
+  double rate_min, rate_max;     /* else round(U(rate_min, rate_max) * plen) events     */
+  int32_t geometric_runs;        /* single-base I/D events get Geometric(0.5) lengths   */
+  double run_frac;               /* probability that an event is an I/D run             */
+  int32_t run_min, run_max;      /* run length U{run_min..run_max}                      */
+} bw_dataset_params_t;
+
+/* Pattern/text lengths of every pair (deterministic in seed and pair index). */
+int bw_dataset_lengths(const bw_dataset_params_t* params, int32_t* pattern_length,
+                       int32_t* text_length);
+/* ASCII sequences written at the given offsets of `seq`. */
+int bw_dataset_generate(const bw_dataset_params_t* params, const int64_t* pattern_offset,
+                        const int64_t* text_offset, char* seq);
+/* Packs ASCII sequences into 2-bit words at word offsets (see bw_batch_t). */
+int bw_pack_2bit(int64_t num_seqs, const char* ascii, const int64_t* offset,
+                 const int32_t* length, const int64_t* word_offset, uint64_t* words);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BIWFA_MI355X_H_ */

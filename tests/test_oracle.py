@@ -1,0 +1,239 @@
+"""Pins the CPU oracle (oracle/biwfa_oracle.c) before anything is compared against it.
+
+* golden vectors: the 305-pair WFA2 utest fixture under 7 penalty sets, full
+  alignment and score-only (tests/wfa.utest.check/test.biwfa*.alg in WFA2-lib);
+* the vendored WFA2-lib built from the reference sources (oracle/_ref), on seeded
+  pairs of every workload and on random pairs, including its step cap;
+* mathematics: a textbook Gotoh DP fixes the optimal score of small pairs, and
+  every CIGAR must spell pattern -> text with M on equal and X on different bases
+  and rescore to the reported score (cigar_check_alignment / cigar_score_gap_affine,
+  alignment/cigar.c:277-303, 617-688);
+* a planted mistake in the oracle (a wrong tie-break) must fail the golden check.
+"""
+import numpy as np
+import pytest
+
+from conftest import PENALTY_SETS, golden_alg
+from oracle.oracle import runs_to_cigar
+
+
+def _random_pairs(seed, n, max_len, p_unrelated=0.2):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m = int(rng.integers(0, max_len + 1))
+        p = "".join(rng.choice(list("ACGT"), m))
+        if rng.random() < p_unrelated:
+            t = "".join(rng.choice(list("ACGT"), int(rng.integers(0, max_len + 1))))
+        else:
+            t = list(p)
+            for _ in range(int(rng.integers(0, max(1, m // 4) + 1))):
+                if not t:
+                    break
+                j = int(rng.integers(0, len(t)))
+                r = rng.random()
+                if r < 0.34:
+                    t[j] = str(rng.choice(list("ACGT")))
+                elif r < 0.67:
+                    t.insert(j, "".join(rng.choice(list("ACGT"), int(rng.integers(1, 6)))))
+                else:
+                    del t[j:j + int(rng.integers(1, 6))]
+            t = "".join(t)
+        out.append((p, t))
+    return out
+
+
+def gotoh_cost(p, t, match, x, o, e):
+    """Textbook Gotoh DP (minimum cost; match <= 0 is a bonus)."""
+    INF = 1 << 40
+    n, m = len(p), len(t)
+    M = [[INF] * (m + 1) for _ in range(n + 1)]
+    I = [[INF] * (m + 1) for _ in range(n + 1)]
+    D = [[INF] * (m + 1) for _ in range(n + 1)]
+    M[0][0] = 0
+    for j in range(1, m + 1):
+        I[0][j] = o + e * j
+        M[0][j] = I[0][j]
+    for i in range(1, n + 1):
+        D[i][0] = o + e * i
+        M[i][0] = D[i][0]
+        for j in range(1, m + 1):
+            I[i][j] = min(M[i][j - 1] + o + e, I[i][j - 1] + e)
+            D[i][j] = min(M[i - 1][j] + o + e, D[i - 1][j] + e)
+            sub = M[i - 1][j - 1] + (match if p[i - 1] == t[j - 1] else x)
+            M[i][j] = min(sub, I[i][j], D[i][j])
+    return M[n][m]
+
+
+def cigar_spells(p, t, cigar):
+    """cigar_check_alignment (alignment/cigar.c:617-688)."""
+    i = j = 0
+    num = 0
+    for ch in cigar:
+        if ch.isdigit():
+            num = num * 10 + int(ch)
+            continue
+        for _ in range(num):
+            if ch == "M":
+                if i >= len(p) or j >= len(t) or p[i] != t[j]:
+                    return False
+                i += 1; j += 1
+            elif ch == "X":
+                if i >= len(p) or j >= len(t) or p[i] == t[j]:
+                    return False
+                i += 1; j += 1
+            elif ch == "I":
+                j += 1
+            elif ch == "D":
+                i += 1
+        num = 0
+    return i == len(p) and j == len(t)
+
+
+def cigar_score(cigar, match, x, o, e):
+    """cigar_score_gap_affine over run-length text."""
+    score, num, last = 0, 0, ""
+    for ch in cigar:
+        if ch.isdigit():
+            num = num * 10 + int(ch)
+            continue
+        if ch == "M":
+            score -= match * num
+        elif ch == "X":
+            score -= x * num
+        else:
+            score -= e * num + (o if last != ch else 0)
+        last, num = ch, 0
+    return score
+
+
+@pytest.mark.parametrize("tag", list(PENALTY_SETS))
+def test_oracle_matches_golden_full(oracle, utest_pairs, tag):
+    gold = golden_alg(tag)
+    r = oracle.align(utest_pairs, PENALTY_SETS[tag])
+    assert set(r.status.tolist()) == {0}
+    for i, (s, c) in enumerate(gold):
+        assert (int(r.alg_score[i]), r.cigar(i)) == (s, c), "pair %d" % i
+
+
+@pytest.mark.parametrize("tag", list(PENALTY_SETS))
+def test_oracle_matches_golden_score_only(oracle, utest_pairs, tag):
+    gold = golden_alg(tag, score_only=True)
+    r = oracle.align(utest_pairs, PENALTY_SETS[tag], score_only=True)
+    assert [int(s) for s in r.alg_score] == [g[0] for g in gold]
+
+
+def test_golden_scores_agree_with_high_memory_wfa(utest_pairs):
+    """BiWFA and classic WFA have the same scores (WFA2 utest compares them by score)."""
+    import os
+    from conftest import UTEST
+    from oracle.oracle import read_alg_file
+    hi = read_alg_file(os.path.join(UTEST, "test.affine.alg"))
+    bi = golden_alg("")
+    assert [h[0] for h in hi] == [b[0] for b in bi]
+
+
+def test_ref_isa_variants_agree(utest_pairs):
+    """The reference's scalar / AVX2 / AVX-512 extend kernels give the same BiWFA output."""
+    from oracle.oracle import Wfa2Ref
+    if not all(Wfa2Ref.available(v) for v in ("v3", "scalar", "native")):
+        pytest.skip("oracle/_ref not built")
+    outs = []
+    for v in ("v3", "scalar", "native"):
+        r = Wfa2Ref(v).align(utest_pairs, (0, 4, 6, 2))
+        outs.append((r.alg_score.tolist(), [r.cigar(i) for i in range(len(utest_pairs))]))
+    assert outs[0] == outs[1] == outs[2]
+
+
+@pytest.mark.parametrize("pen", list(PENALTY_SETS.values()))
+def test_oracle_vs_ref_random(oracle, wfa2ref, pen):
+    pairs = _random_pairs(7, 300, 500)
+    a = oracle.align(pairs, pen)
+    b = wfa2ref.align(pairs, pen)
+    assert a.status.tolist() == b.status.tolist()
+    assert a.alg_score.tolist() == b.alg_score.tolist()
+    assert [a.cigar(i) for i in range(len(pairs))] == [b.cigar(i) for i in range(len(pairs))]
+    s1 = oracle.align(pairs, pen, score_only=True)
+    s2 = wfa2ref.align(pairs, pen, score_only=True)
+    assert s1.alg_score.tolist() == s2.alg_score.tolist()
+
+
+@pytest.mark.parametrize("name,n", [("C1", 100), ("C2", 3000), ("C3", 2), ("C4", 60)])
+def test_oracle_vs_ref_configs(oracle, wfa2ref, name, n):
+    bw = pytest.importorskip("biwfa_gpu_pii2025_amd")
+    ds = bw.generate_dataset(bw.CONFIGS[name], num_pairs=n)
+    pairs = ds.pairs()
+    a = oracle.align(pairs)
+    b = wfa2ref.align(pairs)
+    assert a.alg_score.tolist() == b.alg_score.tolist()
+    assert [a.cigar(i) for i in range(n)] == [b.cigar(i) for i in range(n)]
+
+
+def test_oracle_vs_ref_max_steps(oracle, wfa2ref):
+    pairs = _random_pairs(3, 120, 400)
+    for cap in (10, 40, 120):
+        a = oracle.align(pairs, (0, 4, 6, 2), max_steps=cap)
+        b = wfa2ref.align(pairs, (0, 4, 6, 2), max_steps=cap)
+        assert a.status.tolist() == b.status.tolist()
+        ok = b.status == 0
+        assert a.alg_score[ok].tolist() == b.alg_score[ok].tolist()
+
+
+@pytest.mark.parametrize("pen", [(0, 4, 6, 2), (0, 1, 2, 1), (0, 5, 3, 2), (-3, 5, 3, 2)])
+def test_oracle_optimal_vs_gotoh_and_valid(oracle, pen):
+    """Small pairs: the score is the Gotoh optimum; the CIGAR spells the pair and rescores."""
+    pairs = _random_pairs(5, 150, 45, p_unrelated=0.3)
+    r = oracle.align(pairs, pen)
+    for i, (p, t) in enumerate(pairs):
+        cig = r.cigar(i)
+        if not p and not t:
+            assert cig == ""
+            continue
+        assert cigar_spells(p, t, cig), (p, t, cig)
+        assert cigar_score(cig, *pen) == r.alg_score[i]
+        assert r.alg_score[i] == -gotoh_cost(p, t, *pen), (p, t)
+
+
+def test_oracle_long_pairs_trigger_recursion(oracle):
+    """Pairs long and divergent enough to recurse several levels still spell and rescore."""
+    bw = pytest.importorskip("biwfa_gpu_pii2025_amd")
+    ds = bw.generate_dataset(bw.CONFIGS["C3"], num_pairs=2)
+    r = oracle.align(ds.pairs())
+    for i, (p, t) in enumerate(ds.pairs()):
+        assert cigar_spells(p, t, r.cigar(i))
+        assert cigar_score(r.cigar(i), 0, 4, 6, 2) == r.alg_score[i]
+        assert r.cells[i] > 10_000_000
+
+
+def test_planted_tie_break_mistake_is_caught(utest_pairs, tmp_path):
+    """Swapping the backtrace preference of D-ext over D-open (a plausible slip)
+    must fail the golden comparison -- the fixtures pin the tie-breaks."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    src = open(os.path.join(ROOT, "oracle", "biwfa_oracle.c")).read()
+    planted = src.replace("#define BT_D1_EXT 6\n#define BT_D1_OPEN 5",
+                          "#define BT_D1_EXT 5\n#define BT_D1_OPEN 6")
+    assert planted != src
+    cfile = tmp_path / "planted.c"
+    cfile.write_text(planted)
+    lib = tmp_path / "libplanted.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-I", os.path.join(ROOT, "oracle"),
+                    "-o", str(lib), str(cfile)], check=True)
+    from oracle.oracle import Oracle
+    o = Oracle.__new__(Oracle)
+    import ctypes
+    o.lib = ctypes.CDLL(str(lib), mode=ctypes.RTLD_LOCAL)
+    o.path = str(lib)
+    gold = golden_alg("")
+    r = o.align(utest_pairs, (0, 4, 6, 2))
+    diffs = sum(1 for i, (s, c) in enumerate(gold) if r.cigar(i) != c)
+    assert diffs > 0
+    # the scores stay optimal: only the tie-break moved
+    assert [int(s) for s in r.alg_score] == [g[0] for g in gold]
+
+
+def test_runs_roundtrip():
+    from oracle.oracle import cigar_to_runs
+    for c in ("14M1I86M", "1X", "3D2I1M", ""):
+        assert runs_to_cigar(cigar_to_runs(c)) == c
