@@ -110,7 +110,7 @@ def lib():
         L.bw_dbatch_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(_Result)]
         L.bw_dbatch_free.argtypes = [ctypes.c_void_p]
         L.bw_last_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
-                                     ctypes.POINTER(ctypes.c_float)]
+                                     ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.bw_set_schedule.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.bw_dataset_lengths.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_generate.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p,
@@ -448,9 +448,10 @@ class DeviceBatch:
             raise BiwfaError("bw_synchronize: " + last_error())
 
     def timing(self):
-        t, m = ctypes.c_float(), ctypes.c_float()
-        self.aligner._L.bw_last_timing(self.aligner._h, ctypes.byref(t), ctypes.byref(m))
-        return t.value, m.value
+        """(total_ms, main_kernel_ms, spill_kernel_ms) of the last align(), from HIP events."""
+        t, m, s = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        self.aligner._L.bw_last_timing(self.aligner._h, ctypes.byref(t), ctypes.byref(m), ctypes.byref(s))
+        return t.value, m.value, s.value
 
     def download(self):
         res = BatchResult(self.n, self.cap)

@@ -150,9 +150,9 @@ struct bw_aligner {
   int dev = 0, cu_count = 0;
   size_t lds_per_block_max = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int32_t* d_counters = nullptr;   // [0] LDS queue, [1] spill count, [2] global queue
-  float last_total_ms = 0.f, last_main_ms = 0.f;
+  float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
   // last single-pair result
   int status = 0, score = 0, wf_score = 0;
   std::vector<uint32_t> runs;
@@ -484,6 +484,7 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
                      0, s, p);
   HIP_TRY(hipGetLastError());
   if (!pl.use_lds) HIP_TRY(hipEventRecord(a->ev[2], s));
+  HIP_TRY(hipEventRecord(a->ev[3], s));
   return 0;
 }
 
@@ -491,16 +492,18 @@ int bw_synchronize(bw_aligner_t* a) {
   if (!a) return BW_STATUS_INVALID_ARG;
   HIP_TRY(hipSetDevice(a->dev));
   HIP_TRY(hipStreamSynchronize(a->stream));
-  float total = 0.f, main_ms = 0.f;
-  if (hipEventElapsedTime(&total, a->ev[0], a->ev[2]) == hipSuccess) a->last_total_ms = total;
+  float total = 0.f, main_ms = 0.f, spill_ms = 0.f;
+  if (hipEventElapsedTime(&total, a->ev[0], a->ev[3]) == hipSuccess) a->last_total_ms = total;
   if (hipEventElapsedTime(&main_ms, a->ev[1], a->ev[2]) == hipSuccess) a->last_main_ms = main_ms;
+  if (hipEventElapsedTime(&spill_ms, a->ev[2], a->ev[3]) == hipSuccess) a->last_spill_ms = spill_ms;
   return 0;
 }
 
-int bw_last_timing(const bw_aligner_t* a, float* total_ms, float* main_ms) {
+int bw_last_timing(const bw_aligner_t* a, float* total_ms, float* main_ms, float* spill_ms) {
   if (!a) return BW_STATUS_INVALID_ARG;
   if (total_ms) *total_ms = a->last_total_ms;
   if (main_ms) *main_ms = a->last_main_ms;
+  if (spill_ms) *spill_ms = a->last_spill_ms;
   return 0;
 }
 

@@ -186,8 +186,10 @@ int bw_dbatch_download(bw_aligner_t* aligner, bw_dbatch_t* dbatch, bw_result_t* 
 void bw_dbatch_free(bw_dbatch_t* dbatch);
 
 /* Milliseconds of the last bw_dbatch_align, measured with HIP events on the
- * aligner's stream; *main_ms is the persistent alignment kernel alone. */
-int bw_last_timing(const bw_aligner_t* aligner, float* total_ms, float* main_ms);
+ * aligner's stream (valid after bw_synchronize): *total_ms the whole launch
+ * sequence, *main_ms the first persistent kernel (LDS workspace, every pair),
+ * *spill_ms the second one (global workspace, pairs the first one spilled). */
+int bw_last_timing(const bw_aligner_t* aligner, float* total_ms, float* main_ms, float* spill_ms);
 
 /* ---------------- (iv) scheduling ---------------- */
 
