@@ -13,20 +13,23 @@
 //                         wavefront/wavefront_backtrace.c:38-529
 //
 // Device mapping:
-//   * A persistent grid of 256-thread workgroups; each of the 4 waves is an
-//     independent aligner that pulls pairs from a device work queue (atomic
-//     counter over a schedule order).  No cross-wave synchronisation.
-//   * Diagonals are spread over the 64 lanes; wavefront limits, scores and the
-//     recursion stack are wave-uniform.
-//   * Forward and reverse wavefronts live in modular rings (scope scores x
-//     M/I1/D1 x wcap diagonals, k indexed modulo wcap) -- O(s) memory per pair.
-//     The base case keeps every wavefront of its sub-problem in a bump arena
-//     that reuses the ring storage.
+//   * A persistent grid of workgroups; each wave is an independent aligner that
+//     pulls pairs from a device work queue (atomic counter over a schedule
+//     order).  No cross-wave synchronisation.
+//   * Diagonals are spread over the 64 lanes; scores, limits and the recursion
+//     state are wave-uniform (scalar registers).
+//   * Wavefront limits (lo/hi per component) are kept in vector registers, one
+//     lane per ring slot (or per base-case score), read with v_readlane: no
+//     memory round trip per header access.
+//   * Forward and reverse offsets live in modular rings (scope scores x M/I1/D1 x
+//     wcap diagonals, k indexed modulo wcap) -- O(s) memory per pair.  The base
+//     case keeps every wavefront of its sub-problem in a bump arena that reuses
+//     the ring storage.
 //   * Sequences are 2-bit packed; extension compares 32 bases per 64-bit word
 //     (xor + count-trailing-zeros forward, count-leading-zeros backward).
-//   * Two instantiations: kLds keeps the whole workspace and both sequences in
-//     LDS; a pair that outgrows any LDS capacity is pushed to a spill list and
-//     re-run by the global-workspace instantiation with large capacities.
+//   * Two instantiations: kLds keeps the workspace and both sequences in LDS; a
+//     pair that outgrows an LDS capacity is pushed to a spill list and re-run by
+//     the global-workspace instantiation with large capacities.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
@@ -37,24 +40,43 @@ namespace bw {
 #define DI __device__ __forceinline__
 
 enum { CM = 0, CI = 1, CD = 2 };
-constexpr int kAbsent = INT_MIN;   // base-case header: component not allocated
+constexpr int kAbsent = INT_MAX;   // header lo of a component WFA2 has not allocated
 // uint32 run op codes (SAM encoding with mismatches shown, cigar.c:38-56)
 constexpr int kOpM = 7, kOpX = 8, kOpI = 1, kOpD = 2;
+// Base-case headers held in registers (one lane per score) in the LDS kernel.
+constexpr int kRegScores = kWave;
 
 DI int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+DI int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 DI int imax(int a, int b) { return a >= b ? a : b; }
 DI int imin(int a, int b) { return a <= b ? a : b; }
 
-// Orders this wave's memory writes before its later reads by other lanes.
+// Orders this wave's workspace writes before later reads by other lanes.  LDS
+// operations of one wave execute in issue order, so the LDS kernel only needs
+// the compiler not to move memory operations across this point; the global
+// workspace needs its stores performed before the loads are issued.
+template <bool kLds>
 DI void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  if (kLds) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
 }
 
-DI int wave_max(int v) {
-  for (int d = 32; d >= 1; d >>= 1) v = imax(v, __shfl_xor(v, d, kWave));
-  return v;
+// Max over the wave of non-negative values (DPP row scans, then row broadcasts).
+DI int wave_max_nonneg(int v) {
+  v = imax(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = imax(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = imax(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = imax(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = imax(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = imax(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return rdl(v, kWave - 1);
 }
 
 // A wavefront component as seen by readers: offsets for k in [lo,hi] live at
@@ -82,6 +104,7 @@ struct BP {                 // wavefront_bialigner.h:43-54
   int score, sf, sr, kf, kr, of, orr, comp;
 };
 
+template <bool kLds>
 struct Ctx {
   int32_t* ws;
   const uint64_t* P;
@@ -91,6 +114,9 @@ struct Ctx {
   int x, o, e, scope, max_steps;
   int um, ux, uo, ue;
   int arena_top;
+  // headers in registers: ring (lane = ring*scope + slot) and base case (lane = score)
+  int rlo[3], rhi[3];
+  int blo[3], bhi[3], bkb[3];
   uint32_t* out;
   int out_cap, out_n, pend_op, pend_len, alg_score;
   int spill;      // a workspace capacity was exceeded
@@ -101,61 +127,102 @@ struct Ctx {
 // Upper bound of the optimal score of a plen x tlen sub-problem plus the
 // slack the breakpoint search may run past it; a search that goes beyond it
 // can only be a defect, and is stopped with kStGuard instead of looping.
-DI int score_guard(const Ctx& c, int plen, int tlen) {
+template <bool kLds>
+DI int score_guard(const Ctx<kLds>& c, int plen, int tlen) {
   const int mn = imin(plen, tlen), df = imax(plen, tlen) - mn;
   return mn * c.x + c.o + df * c.e + c.o + 2 * c.scope + 16;
 }
 
-// ---------------------------------------------------------------------------
-// storage addressing
-
-DI int ring_hdr(const Ctx& c, int ring, int slot, int comp) {
-  return c.L.ring_hdr + ((ring * c.scope + slot) * 3 + comp) * 4;
+// Register-array access with a run-time component index.  The values pass
+// through an empty asm so that the selection stays a selection of values: a
+// select of the three addresses would pin the aligner state to scratch memory.
+DI int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
 }
-DI int ring_base(const Ctx& c, int ring, int slot, int comp) {
+DI int sel3(const int (&a)[3], int comp) {
+  const int a0 = opaque(a[0]), a1 = opaque(a[1]), a2 = opaque(a[2]);
+  return comp == CM ? a0 : (comp == CI ? a1 : a2);
+}
+
+// ring slot of (score - delta) given the slot of score; 0 <= delta < scope
+DI int slot_back(int slot, int delta, int scope) {
+  const int r = slot - delta;
+  return r < 0 ? r + scope : r;
+}
+
+// ---------------------------------------------------------------------------
+// headers
+
+template <bool kLds>
+DI int ring_base(const Ctx<kLds>& c, int ring, int slot, int comp) {
   return c.L.data + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
 }
-DI int base_hdr(const Ctx& c, int score, int comp) { return c.L.bhdr + (score * 3 + comp) * 3; }
 
-// Component `comp` at `score` (WFA2's wavefront_compute_get_*wavefront,
-// wavefront_compute.c:266-305).  *present reports a non-NULL WFA2 pointer
-// (possibly with lo > hi); the returned view is null when absent or empty.
-DI Wf get_wf(const Ctx& c, const Dir& d, int score, int comp, int* present) {
+// Component `comp` at `score` (ring slot `slot`), WFA2's
+// wavefront_compute_get_*wavefront (wavefront_compute.c:266-305).  *present
+// reports a non-NULL WFA2 pointer (possibly with lo > hi).
+template <bool kLds>
+DI Wf get_wf(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, int* present) {
   *present = 0;
   if (score < 0) return wf_null();
   if (d.ring >= 0) {
-    const int slot = score % c.scope;
-    const int h = ring_hdr(c, d.ring, slot, comp);
-    if (!uni(c.ws[h + 2])) return wf_null();
+    const int j = d.ring * c.scope + slot;
+    const int lo = rdl(sel3(c.rlo, comp), j);
+    if (lo == kAbsent) return wf_null();
     *present = 1;
-    const int lo = uni(c.ws[h]), hi = uni(c.ws[h + 1]);
+    const int hi = rdl(sel3(c.rhi, comp), j);
     if (lo > hi) return Wf{lo, hi, 0, 0};
     return Wf{lo, hi, ring_base(c, d.ring, slot, comp), c.L.wcap - 1};
   }
-  const int h = base_hdr(c, score, comp);
-  const int kb = uni(c.ws[h + 2]);
-  if (kb == kAbsent) return wf_null();
+  int lo, hi, kb;
+  if (kLds) {
+    lo = rdl(sel3(c.blo, comp), score);
+    hi = rdl(sel3(c.bhi, comp), score);
+    kb = rdl(sel3(c.bkb, comp), score);
+  } else {
+    const int h = c.L.bhdr + (score * 3 + comp) * 3;
+    lo = uni(c.ws[h]);
+    hi = uni(c.ws[h + 1]);
+    kb = uni(c.ws[h + 2]);
+  }
+  if (lo == kAbsent) return wf_null();
   *present = 1;
-  return Wf{uni(c.ws[h]), uni(c.ws[h + 1]), kb, -1};
+  return Wf{lo, hi, kb, -1};
 }
-DI Wf get_in(const Ctx& c, const Dir& d, int score, int comp) {
+
+template <bool kLds>
+DI Wf get_in(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp) {
   int present;
-  Wf w = get_wf(c, d, score, comp, &present);
+  Wf w = get_wf(c, d, score, slot, comp, &present);
   if (w.lo > w.hi) return wf_null();
   return w;
 }
-DI void set_hdr(Ctx& c, const Dir& d, int score, int comp, int lo, int hi, int base_or_absent,
-                int present) {
+
+DI void lane_set(int (&a)[3], int comp, int lane, int j, int v) {
+  const bool me = lane == j;
+  const int a0 = opaque(a[0]), a1 = opaque(a[1]), a2 = opaque(a[2]);
+  a[0] = (me && comp == CM) ? v : a0;
+  a[1] = (me && comp == CI) ? v : a1;
+  a[2] = (me && comp == CD) ? v : a2;
+}
+
+// lo == kAbsent marks an unallocated component
+template <bool kLds>
+DI void set_hdr(Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, int lo, int hi, int kb) {
   if (d.ring >= 0) {
-    const int h = ring_hdr(c, d.ring, score % c.scope, comp);
-    c.ws[h] = lo;
-    c.ws[h + 1] = hi;
-    c.ws[h + 2] = present;
+    const int j = d.ring * c.scope + slot;
+    lane_set(c.rlo, comp, c.lane, j, lo);
+    lane_set(c.rhi, comp, c.lane, j, hi);
+  } else if (kLds) {
+    lane_set(c.blo, comp, c.lane, score, lo);
+    lane_set(c.bhi, comp, c.lane, score, hi);
+    lane_set(c.bkb, comp, c.lane, score, kb);
   } else {
-    const int h = base_hdr(c, score, comp);
+    const int h = c.L.bhdr + (score * 3 + comp) * 3;
     c.ws[h] = lo;
     c.ws[h + 1] = hi;
-    c.ws[h + 2] = present ? base_or_absent : kAbsent;
+    c.ws[h + 2] = kb;
   }
 }
 
@@ -171,7 +238,8 @@ DI uint64_t window(const uint64_t* s, int pos) {       // bases [pos, pos+32)
 
 // Longest common extension from (v,h) of the aligner's sub-problem, 32 bases
 // per 64-bit compare, clamped to the sub-problem's end.
-DI int lcp(const Ctx& c, const Dir& d, int v, int h) {
+template <bool kLds>
+DI int lcp(const Ctx<kLds>& c, const Dir& d, int v, int h) {
   const int lim = imin(d.plen - v, d.tlen - h);
   int n = 0;
   if (!d.rev) {
@@ -195,20 +263,22 @@ DI int lcp(const Ctx& c, const Dir& d, int v, int h) {
 
 // ---------------------------------------------------------------------------
 // compute: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
+// `slot` is score % scope for ring aligners.
 
-DI void compute_step(Ctx& c, Dir& d, int s) {
-  const Wf mm = get_in(c, d, s - c.x, CM);
-  const Wf mo = get_in(c, d, s - c.o - c.e, CM);
-  const Wf ie = get_in(c, d, s - c.e, CI);
-  const Wf de = get_in(c, d, s - c.e, CD);
+template <bool kLds>
+DI void compute_step(Ctx<kLds>& c, Dir& d, int s, int slot) {
+  const int sc = c.scope;
+  const Wf mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
+  const Wf mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
+  const Wf ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
+  const Wf de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
   if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return; }
   const bool nmm = mm.lo > mm.hi, nmo = mo.lo > mo.hi, nie = ie.lo > ie.hi, nde = de.lo > de.hi;
   if (nmm && nmo && nie && nde) {               // null step
     d.null_steps++;
-    set_hdr(c, d, s, CM, 1, -1, 0, 0);
-    set_hdr(c, d, s, CI, 1, -1, 0, 0);
-    set_hdr(c, d, s, CD, 1, -1, 0, 0);
-    wave_sync();
+    set_hdr(c, d, s, slot, CM, kAbsent, 0, 0);
+    set_hdr(c, d, s, slot, CI, kAbsent, 0, 0);
+    set_hdr(c, d, s, slot, CD, kAbsent, 0, 0);
     return;
   }
   d.null_steps = 0;
@@ -222,7 +292,6 @@ DI void compute_step(Ctx& c, Dir& d, int s) {
   int bm, bi, bd, mask;
   if (d.ring >= 0) {
     if (width > c.L.wcap) { c.spill = 1; return; }
-    const int slot = s % c.scope;
     bm = ring_base(c, d.ring, slot, CM);
     bi = ring_base(c, d.ring, slot, CI);
     bd = ring_base(c, d.ring, slot, CD);
@@ -266,28 +335,30 @@ DI void compute_step(Ctx& c, Dir& d, int s) {
   if (lm == INT_MIN) { fm = lo; lm = lo - 1; }
   if (li == INT_MIN) { fi = lo; li = lo - 1; }
   if (ld == INT_MIN) { fd = lo; ld = lo - 1; }
-  set_hdr(c, d, s, CM, fm, lm, bm, 1);
-  set_hdr(c, d, s, CI, fi, li, bi, has_i);
-  set_hdr(c, d, s, CD, fd, ld, bd, has_d);
-  wave_sync();
+  set_hdr(c, d, s, slot, CM, fm, lm, bm);
+  set_hdr(c, d, s, slot, CI, has_i ? fi : kAbsent, li, bi);
+  set_hdr(c, d, s, slot, CD, has_d ? fd : kAbsent, ld, bd);
+  wave_sync<kLds>();
 }
 
 // ---------------------------------------------------------------------------
 // extend + termination (wavefront_extend.c:86-211, wavefront_termination.c:37-114)
 
-DI bool terminated(const Ctx& c, const Dir& d, int s) {
+template <bool kLds>
+DI bool terminated(const Ctx<kLds>& c, const Dir& d, int s, int slot) {
   const int k_end = d.tlen - d.plen;
   int present;
-  const Wf w = get_wf(c, d, s, d.ce, &present);
+  const Wf w = get_wf(c, d, s, slot, d.ce, &present);
   if (!present || w.lo > k_end || k_end > w.hi) return false;
   return uni(c.ws[w.base + (k_end & w.mask)]) >= d.tlen;
 }
 
 // Returns 1 when the aligner is done; *max_ak gets the max antidiagonal.
-DI int extend_step(Ctx& c, Dir& d, int s, int* max_ak) {
+template <bool kLds>
+DI int extend_step(Ctx<kLds>& c, Dir& d, int s, int slot, int* max_ak) {
   *max_ak = 0;
   int present;
-  const Wf m = get_wf(c, d, s, CM, &present);
+  const Wf m = get_wf(c, d, s, slot, CM, &present);
   if (!present) {
     if (d.null_steps > c.scope) {
       d.status = kStEndUnreachable;
@@ -309,9 +380,9 @@ DI int extend_step(Ctx& c, Dir& d, int s, int* max_ak) {
       }
     }
   }
-  wave_sync();
-  lmax = uni(wave_max(lmax));
-  if (terminated(c, d, s)) {
+  wave_sync<kLds>();
+  lmax = wave_max_nonneg(lmax);
+  if (terminated(c, d, s, slot)) {
     d.status = kStEndReached;
     d.status_score = s;
     return 1;
@@ -325,8 +396,9 @@ DI int extend_step(Ctx& c, Dir& d, int s, int* max_ak) {
 
 // First k0 (ascending) of the overlapping interval that satisfies the overlap
 // condition; component `comp` (CD/CI: indel-to-indel, CM: M-to-M).
-DI void scan_breakpoint(const Ctx& c, bool forward, int s0, int s1, const Wf& w0, const Wf& w1,
-                        int comp, int tl, int pl, BP& bp) {
+template <bool kLds>
+DI void scan_breakpoint(const Ctx<kLds>& c, bool forward, int s0, int s1, const Wf& w0,
+                        const Wf& w1, int comp, int tl, int pl, BP& bp) {
   const int lo0 = w0.lo, hi0 = w0.hi;
   const int lo1 = tl - pl - w1.hi, hi1 = tl - pl - w1.lo;
   if (hi1 < lo0 || hi0 < lo1) return;
@@ -349,7 +421,7 @@ DI void scan_breakpoint(const Ctx& c, bool forward, int s0, int s1, const Wf& w0
     const uint64_t b = __ballot(ok);
     if (b) {
       const int j = __builtin_ctzll(b);
-      const int a0 = uni(__shfl(o0, j, kWave)), a1 = uni(__shfl(o1, j, kWave));
+      const int a0 = rdl(o0, j), a1 = rdl(o1, j);
       const int kk0 = kb + j, kk1 = tl - pl - kk0;
       if (forward) {
         bp.sf = s0; bp.sr = s1; bp.kf = kk0; bp.kr = kk1; bp.of = a0; bp.orr = a1;
@@ -363,73 +435,73 @@ DI void scan_breakpoint(const Ctx& c, bool forward, int s0, int s1, const Wf& w0
   }
 }
 
-DI void overlap(const Ctx& c, const Dir& a0, const Dir& a1, int s0, int s1, bool forward,
-                BP& bp) {
+template <bool kLds>
+DI void overlap(const Ctx<kLds>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int s1,
+                int slot1, bool forward, BP& bp) {
   int pm0, pi0, pd0;
-  const Wf m0 = get_wf(c, a0, s0, CM, &pm0);
+  const Wf m0 = get_wf(c, a0, s0, slot0, CM, &pm0);
   if (!pm0) return;
-  const Wf i0 = get_wf(c, a0, s0, CI, &pi0);
-  const Wf d0 = get_wf(c, a0, s0, CD, &pd0);
+  const Wf i0 = get_wf(c, a0, s0, slot0, CI, &pi0);
+  const Wf d0 = get_wf(c, a0, s0, slot0, CD, &pd0);
   const int tl = a0.tlen, pl = a0.plen;
   for (int i = 0; i < c.scope; ++i) {
     const int si = s1 - i;
     if (si < 0) break;
     if (s0 + si - c.o >= bp.score) continue;
+    const int sl = slot_back(slot1, i, c.scope);
     int p1;
-    const Wf d1 = get_wf(c, a1, si, CD, &p1);
+    const Wf d1 = get_wf(c, a1, si, sl, CD, &p1);
     if (pd0 && p1) scan_breakpoint(c, forward, s0, si, d0, d1, CD, tl, pl, bp);
-    const Wf i1 = get_wf(c, a1, si, CI, &p1);
+    const Wf i1 = get_wf(c, a1, si, sl, CI, &p1);
     if (pi0 && p1) scan_breakpoint(c, forward, s0, si, i0, i1, CI, tl, pl, bp);
     if (s0 + si >= bp.score) continue;
-    const Wf m1 = get_wf(c, a1, si, CM, &p1);
+    const Wf m1 = get_wf(c, a1, si, sl, CM, &p1);
     if (p1) scan_breakpoint(c, forward, s0, si, m0, m1, CM, tl, pl, bp);
   }
 }
 
 // wavefront_aligner_init_wf (wavefront_aligner.c:311-383) for a ring aligner.
-DI void ring_init(Ctx& c, Dir& d) {
-  for (int i = c.lane; i < c.scope * 3; i += kWave) {
-    const int h = c.L.ring_hdr + (d.ring * c.scope * 3 + i) * 4;
-    c.ws[h + 2] = 0;
-  }
-  wave_sync();
-  const int h = ring_hdr(c, d.ring, 0, d.cb);
-  c.ws[h] = 0;
-  c.ws[h + 1] = 0;
-  c.ws[h + 2] = 1;
+template <bool kLds>
+DI void ring_init(Ctx<kLds>& c, Dir& d) {
+  const bool mine = c.lane >= d.ring * c.scope && c.lane < (d.ring + 1) * c.scope;
+  for (int q = 0; q < 3; ++q) c.rlo[q] = mine ? kAbsent : c.rlo[q];
+  set_hdr(c, d, 0, 0, d.cb, 0, 0, 0);
   c.ws[ring_base(c, d.ring, 0, d.cb)] = 0;
   d.null_steps = 0;
   d.status = kStOk;
   d.status_score = 0;
-  wave_sync();
+  wave_sync<kLds>();
 }
 
+DI int next_slot(int slot, int scope) { return slot + 1 == scope ? 0 : slot + 1; }
+
 // wavefront_bialign_find_breakpoint (wavefront_bialign.c:411-519)
-DI int find_breakpoint(Ctx& c, Dir& f, Dir& r, BP& bp) {
+template <bool kLds>
+DI int find_breakpoint(Ctx<kLds>& c, Dir& f, Dir& r, BP& bp) {
   ring_init(c, f);
   ring_init(c, r);
   const int max_antidiagonal = f.plen + f.tlen - 1;
-  int sf = 0, sr = 0, f_max = 0, r_max = 0, ak = 0;
+  int sf = 0, sr = 0, qf = 0, qr = 0, f_max = 0, r_max = 0, ak = 0;
   bp.score = INT_MAX;
   c.guard = score_guard(c, f.plen, f.tlen);
-  if (extend_step(c, f, sf, &f_max)) return f.status;
-  if (extend_step(c, r, sr, &r_max)) return r.status;
+  if (extend_step(c, f, sf, qf, &f_max)) return f.status;
+  if (extend_step(c, r, sr, qr, &r_max)) return r.status;
   bool last_forward = false;
   for (;;) {
     if (f_max + r_max >= max_antidiagonal) break;
     if (sf + sr > c.guard) return kStGuard;
-    ++sf;
-    compute_step(c, f, sf);
+    ++sf; qf = next_slot(qf, c.scope);
+    compute_step(c, f, sf, qf);
     if (c.spill) return kStCapacity;
-    const int fq = extend_step(c, f, sf, &ak);
+    const int fq = extend_step(c, f, sf, qf, &ak);
     if (f_max < ak) f_max = ak;
     last_forward = true;
     if (fq) return f.status;
     if (f_max + r_max >= max_antidiagonal) break;
-    ++sr;
-    compute_step(c, r, sr);
+    ++sr; qr = next_slot(qr, c.scope);
+    compute_step(c, r, sr, qr);
     if (c.spill) return kStCapacity;
-    const int rq = extend_step(c, r, sr, &ak);
+    const int rq = extend_step(c, r, sr, qr, &ak);
     if (r_max < ak) r_max = ak;
     last_forward = false;
     if (rq) return r.status;
@@ -439,20 +511,20 @@ DI int find_breakpoint(Ctx& c, Dir& f, Dir& r, BP& bp) {
     if (last_forward) {
       const int min_r = (sr > c.scope - 1) ? sr - (c.scope - 1) : 0;
       if (sf + min_r - c.o >= bp.score) break;
-      overlap(c, f, r, sf, sr, true, bp);
-      ++sr;
-      compute_step(c, r, sr);
+      overlap(c, f, r, sf, qf, sr, qr, true, bp);
+      ++sr; qr = next_slot(qr, c.scope);
+      compute_step(c, r, sr, qr);
       if (c.spill) return kStCapacity;
-      if (extend_step(c, r, sr, &ak)) return r.status;
+      if (extend_step(c, r, sr, qr, &ak)) return r.status;
     }
     const int min_f = (sf > c.scope - 1) ? sf - (c.scope - 1) : 0;
     if (min_f + sr - c.o >= bp.score) break;
     if (sf + sr > c.guard) return kStGuard;
-    overlap(c, r, f, sr, sf, false, bp);
-    ++sf;
-    compute_step(c, f, sf);
+    overlap(c, r, f, sr, qr, sf, qf, false, bp);
+    ++sf; qf = next_slot(qf, c.scope);
+    compute_step(c, f, sf, qf);
     if (c.spill) return kStCapacity;
-    if (extend_step(c, f, sf, &ak)) return f.status;
+    if (extend_step(c, f, sf, qf, &ak)) return f.status;
     if (sr + sf >= c.max_steps) return kStMaxSteps;
     last_forward = true;
   }
@@ -462,7 +534,8 @@ DI int find_breakpoint(Ctx& c, Dir& f, Dir& r, BP& bp) {
 // ---------------------------------------------------------------------------
 // output runs
 
-DI void out_flush(Ctx& c) {
+template <bool kLds>
+DI void out_flush(Ctx<kLds>& c) {
   if (c.pend_len <= 0) return;
   if (c.out_n >= c.out_cap) {
     c.cap_err = 1;
@@ -471,15 +544,14 @@ DI void out_flush(Ctx& c) {
     c.out_n++;
   }
   // cigar_score_gap_affine (cigar.c:277-303) over maximal runs
-  switch (c.pend_op) {
-    case kOpM: c.alg_score -= c.um * c.pend_len; break;
-    case kOpX: c.alg_score -= c.ux * c.pend_len; break;
-    default: c.alg_score -= c.uo + c.ue * c.pend_len; break;
-  }
+  if (c.pend_op == kOpM) c.alg_score -= c.um * c.pend_len;
+  else if (c.pend_op == kOpX) c.alg_score -= c.ux * c.pend_len;
+  else c.alg_score -= c.uo + c.ue * c.pend_len;
   c.pend_len = 0;
   c.pend_op = -1;
 }
-DI void out_append(Ctx& c, int op, int n) {
+template <bool kLds>
+DI void out_append(Ctx<kLds>& c, int op, int n) {
   if (n <= 0) return;
   if (op == c.pend_op) { c.pend_len += n; return; }
   out_flush(c);
@@ -491,7 +563,8 @@ DI void out_append(Ctx& c, int op, int n) {
 struct RevRuns {
   int n, op, len;
 };
-DI void rev_emit(Ctx& c, RevRuns& rr, int op, int n) {
+template <bool kLds>
+DI void rev_emit(Ctx<kLds>& c, RevRuns& rr, int op, int n) {
   if (n <= 0) return;
   if (op == rr.op) { rr.len += n; return; }
   if (rr.len > 0) {
@@ -508,20 +581,19 @@ DI void rev_emit(Ctx& c, RevRuns& rr, int op, int n) {
 
 constexpr int kBtM = 9, kBtD1Ext = 6, kBtD1Open = 5, kBtI1Ext = 2, kBtI1Open = 1;
 
-DI long long bt_src(const Ctx& c, int score, int comp, int k, int add, int type) {
-  if (score < 0) return (long long)kOffsetNull;
-  const int h = base_hdr(c, score, comp);
-  const int kb = uni(c.ws[h + 2]);
-  if (kb == kAbsent) return (long long)kOffsetNull;
-  const int lo = uni(c.ws[h]), hi = uni(c.ws[h + 1]);
-  if (!(lo <= k && k <= hi)) return (long long)kOffsetNull;
-  return ((long long)(uni(c.ws[kb + k]) + add) << 4) | type;
+template <bool kLds>
+DI long long bt_src(const Ctx<kLds>& c, const Dir& b, int score, int comp, int k, int add, int type) {
+  int present;
+  const Wf w = get_wf(c, b, score, 0, comp, &present);
+  if (!present || !(w.lo <= k && k <= w.hi)) return (long long)kOffsetNull;
+  return ((long long)(uni(c.ws[w.base + k]) + add) << 4) | type;
 }
 DI long long max64(long long a, long long b) { return a >= b ? a : b; }
 
 // wavefront_backtrace_affine (wavefront_backtrace.c:320-529); appends the
 // operations to the output in forward order.  Returns 0 or -1 on a trace error.
-DI int backtrace(Ctx& c, const Dir& b, int score) {
+template <bool kLds>
+DI int backtrace(Ctx<kLds>& c, const Dir& b, int score) {
   int matrix = b.ce;
   int k = b.tlen - b.plen, off = b.tlen;
   int h = off, v = off - k;
@@ -536,18 +608,18 @@ DI int backtrace(Ctx& c, const Dir& b, int score) {
     const int gap_extend1 = score - c.e;
     long long max_all;
     if (matrix == CM) {
-      const long long misms = bt_src(c, mismatch, CM, k, 1, kBtM);
-      const long long io = bt_src(c, gap_open1, CM, k - 1, 1, kBtI1Open);
-      const long long ix = bt_src(c, gap_extend1, CI, k - 1, 1, kBtI1Ext);
-      const long long dop = bt_src(c, gap_open1, CM, k + 1, 0, kBtD1Open);
-      const long long dx = bt_src(c, gap_extend1, CD, k + 1, 0, kBtD1Ext);
+      const long long misms = bt_src(c, b, mismatch, CM, k, 1, kBtM);
+      const long long io = bt_src(c, b, gap_open1, CM, k - 1, 1, kBtI1Open);
+      const long long ix = bt_src(c, b, gap_extend1, CI, k - 1, 1, kBtI1Ext);
+      const long long dop = bt_src(c, b, gap_open1, CM, k + 1, 0, kBtD1Open);
+      const long long dx = bt_src(c, b, gap_extend1, CD, k + 1, 0, kBtD1Ext);
       max_all = max64(misms, max64(max64(io, ix), max64(dop, dx)));
     } else if (matrix == CI) {
-      max_all = max64(bt_src(c, gap_open1, CM, k - 1, 1, kBtI1Open),
-                      bt_src(c, gap_extend1, CI, k - 1, 1, kBtI1Ext));
+      max_all = max64(bt_src(c, b, gap_open1, CM, k - 1, 1, kBtI1Open),
+                      bt_src(c, b, gap_extend1, CI, k - 1, 1, kBtI1Ext));
     } else {
-      max_all = max64(bt_src(c, gap_open1, CM, k + 1, 0, kBtD1Open),
-                      bt_src(c, gap_extend1, CD, k + 1, 0, kBtD1Ext));
+      max_all = max64(bt_src(c, b, gap_open1, CM, k + 1, 0, kBtD1Open),
+                      bt_src(c, b, gap_extend1, CD, k + 1, 0, kBtD1Ext));
     }
     if (max_all < 0) break;
     if (matrix == CM) {
@@ -583,7 +655,7 @@ DI int backtrace(Ctx& c, const Dir& b, int score) {
     return -1;
   }
   if (c.spill) return 0;
-  wave_sync();
+  wave_sync<kLds>();
   // forward order: the pending run first, then the stored runs backwards
   out_append(c, rr.op, rr.len);
   for (int i = rr.n - 1; i >= 0; --i) {
@@ -595,27 +667,28 @@ DI int backtrace(Ctx& c, const Dir& b, int score) {
 
 // Unidirectional WFA over the sub-problem with all wavefronts kept
 // (wavefront_unialign.c:242-273).  Returns kStOk or kStUnattainable.
-DI int base_case(Ctx& c, int pb, int pe, int tb, int te, int cb, int ce, int* score_out) {
+template <bool kLds>
+DI int base_case(Ctx<kLds>& c, int pb, int pe, int tb, int te, int cb, int ce, int* score_out) {
   Dir b{pb, pe - pb, tb, te - tb, 0, -1, cb, ce, 0, kStOk, 0};
   // score 0: the begin component at k=0 (wavefront_aligner.c:311-383)
   c.arena_top = 1;
-  set_hdr(c, b, 0, CM, 0, 0, c.L.data, cb == CM);
-  set_hdr(c, b, 0, CI, 0, 0, c.L.data, cb == CI);
-  set_hdr(c, b, 0, CD, 0, 0, c.L.data, cb == CD);
+  set_hdr(c, b, 0, 0, CM, cb == CM ? 0 : kAbsent, 0, c.L.data);
+  set_hdr(c, b, 0, 0, CI, cb == CI ? 0 : kAbsent, 0, c.L.data);
+  set_hdr(c, b, 0, 0, CD, cb == CD ? 0 : kAbsent, 0, c.L.data);
   c.ws[c.L.data] = 0;
-  wave_sync();
+  wave_sync<kLds>();
   int score = 0, ak;
   const int guard = score_guard(c, b.plen, b.tlen);
   for (;;) {
     if (score > guard) return kStGuard;
-    if (extend_step(c, b, score, &ak)) {
+    if (extend_step(c, b, score, 0, &ak)) {
       if (b.status != kStEndReached) return kStUnattainable;
       if (backtrace(c, b, score) != 0) return kStUnattainable;
       *score_out = score;
       return kStOk;
     }
     ++score;
-    compute_step(c, b, score);
+    compute_step(c, b, score, 0);
     if (c.spill) return kStCapacity;
     if (score >= c.max_steps) return kStUnattainable;   // reported as unattainable by the base
   }
@@ -627,21 +700,24 @@ DI int base_case(Ctx& c, int pb, int pe, int tb, int te, int cb, int ce, int* sc
 struct Sub {
   int pb, pe, tb, te, cb, ce, rem, lvl;
 };
-DI void push(Ctx& c, int& sp, const Sub& s) {
+template <bool kLds>
+DI void push(Ctx<kLds>& c, int& sp, const Sub& s) {
   if (sp >= c.L.scap) { c.spill = 1; return; }
   int* e = c.ws + c.L.stack + sp * 8;
   e[0] = s.pb; e[1] = s.pe; e[2] = s.tb; e[3] = s.te;
   e[4] = s.cb; e[5] = s.ce; e[6] = s.rem; e[7] = s.lvl;
   ++sp;
 }
-DI Sub pop(Ctx& c, int& sp) {
+template <bool kLds>
+DI Sub pop(Ctx<kLds>& c, int& sp) {
   --sp;
   const int* e = c.ws + c.L.stack + sp * 8;
   return Sub{uni(e[0]), uni(e[1]), uni(e[2]), uni(e[3]), uni(e[4]), uni(e[5]), uni(e[6]), uni(e[7])};
 }
 
 // Aligns the staged pair.  Returns the final WFA status (wavefront_bialign.c:722-729).
-DI int align_pair(Ctx& c, int plen, int tlen, int score_only, int* wf_score) {
+template <bool kLds>
+DI int align_pair(Ctx<kLds>& c, int plen, int tlen, int score_only, int* wf_score) {
   *wf_score = INT_MIN;
   if (score_only) {
     // wavefront_bialign_compute_score (wavefront_bialign.c:662-699)
@@ -661,7 +737,7 @@ DI int align_pair(Ctx& c, int plen, int tlen, int score_only, int* wf_score) {
   int sp = 0;
   const bool min_length = imax(plen, tlen) <= kFallbackMinLength;
   push(c, sp, Sub{0, plen, 0, tlen, CM, CM, min_length ? 0 : INT_MAX, 0});
-  wave_sync();
+  wave_sync<kLds>();
   int status = kStOk;
   int iters = 0;
   const int max_iters = 4 * (plen + tlen) + 64;
@@ -705,7 +781,7 @@ DI int align_pair(Ctx& c, int plen, int tlen, int score_only, int* wf_score) {
     push(c, sp, Sub{s.pb + bv, s.pe, s.tb + bh, s.te, bp.comp, s.ce, bp.sr, s.lvl + 1});
     push(c, sp, Sub{s.pb, s.pb + bv, s.tb, s.tb + bh, s.cb, bp.comp, bp.sf, s.lvl + 1});
     if (c.spill) return kStCapacity;
-    wave_sync();
+    wave_sync<kLds>();
   }
   out_flush(c);
   if (status == kStOk) return 0;
@@ -721,14 +797,14 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
   extern __shared__ int32_t lds[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
+  const int waves_per_block = blockDim.x / kWave;
   int32_t* ws = kLds ? lds + wv * p.L.total
-                     : p.gws + (int64_t)(blockIdx.x * kWavesPerBlock + wv) * p.gws_stride;
+                     : p.gws + (int64_t)(blockIdx.x * waves_per_block + wv) * p.gws_stride;
   const int count = p.work_count ? uni(*(volatile const int32_t*)p.work_count) : p.num_work;
   for (;;) {
     int base = 0;
     if (lane == 0) base = atomicAdd(p.queue, p.chunk);
-    base = __shfl(base, 0, kWave);
-    base = uni(base);
+    base = uni(__shfl(base, 0, kWave));
     if (base >= count) break;
     const int end = imin(base + p.chunk, count);
     for (int w = base; w < end; ++w) {
@@ -737,7 +813,7 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
       const int plen = uni(p.p_len[pair]), tlen = uni(p.t_len[pair]);
       const uint64_t* gP = p.words + p.p_woff[pair];
       const uint64_t* gT = p.words + p.t_woff[pair];
-      Ctx c;
+      Ctx<kLds> c;
       c.ws = ws;
       c.lane = lane;
       c.L = p.L;
@@ -745,10 +821,14 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
       c.max_steps = p.max_steps;
       c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
       c.arena_top = 0;
+      for (int q = 0; q < 3; ++q) {
+        c.rlo[q] = kAbsent; c.rhi[q] = 0;
+        c.blo[q] = kAbsent; c.bhi[q] = 0; c.bkb[q] = 0;
+      }
       c.out = p.runs + p.runs_off[pair];
       c.out_cap = (int)(p.runs_off[pair + 1] - p.runs_off[pair]);
       c.out_n = 0; c.pend_op = -1; c.pend_len = 0; c.alg_score = 0;
-      c.spill = 0; c.cap_err = 0;
+      c.spill = 0; c.cap_err = 0; c.guard = 0;
       if (kLds) {
         // stage both sequences: [pad][words...][pad][pad] (the pads keep the
         // 64-bit windows in bounds at either end)
@@ -766,7 +846,7 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
           }
           c.P = sP + 1;
           c.T = sT + 1;
-          wave_sync();
+          wave_sync<kLds>();
         }
       } else {
         c.P = gP;

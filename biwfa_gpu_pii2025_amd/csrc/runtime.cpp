@@ -61,13 +61,14 @@ int next_pow2(int v) {
   return p;
 }
 
-// Per-wave workspace layout (int32 units).
+// Per-wave workspace layout (int32 units).  Wavefront limits live in registers;
+// only the global-workspace kernel keeps base-case headers in memory
+// (mem_base_hdr), because its base cases may exceed one register lane per score.
 bw::WsLayout make_layout(int scope, int wcap, int arena_min, int hcap, int rcap, int scap,
-                         int seq_words) {
+                         int seq_words, bool mem_base_hdr) {
   bw::WsLayout L{};
   int off = 0;
-  L.ring_hdr = off;
-  off += 2 * scope * 3 * 4;
+  L.ring_hdr = 0;
   L.data = off;
   L.wcap = wcap;
   const int ring_ints = 2 * scope * 3 * wcap;
@@ -75,7 +76,7 @@ bw::WsLayout make_layout(int scope, int wcap, int arena_min, int hcap, int rcap,
   off += L.arena_cap;
   L.bhdr = off;
   L.hcap = hcap;
-  off += hcap * 3 * 3;
+  if (mem_base_hdr) off += hcap * 3 * 3;
   L.runs_tmp = off;
   L.rcap = rcap;
   off += rcap;
@@ -381,7 +382,7 @@ static Plan make_plan(const bw_aligner_t* a, const bw_dbatch_t* b) {
   if (maxlen <= 2048) {
     const int wcap = (maxlen <= 400) ? 32 : 64;
     const int seq_words = (maxlen + 31) / 32 + 3;
-    pl.L_lds = make_layout(scope, wcap, 0, 64, 128, 16, seq_words);
+    pl.L_lds = make_layout(scope, wcap, 0, 64, 128, 16, seq_words, false);
     const size_t bytes_per_wave = (size_t)pl.L_lds.total * 4;
     int wpb = 4;
     while (wpb > 1 && bytes_per_wave * wpb > 64 * 1024) wpb >>= 1;
@@ -404,7 +405,7 @@ static Plan make_plan(const bw_aligner_t* a, const bw_dbatch_t* b) {
   const int64_t arena = std::min<int64_t>((int64_t)3 * (smax + 1) * width, (int64_t)1 << 28);
   const int steps = std::min(pt, smax / std::max(1, std::min(q.x, q.e)) + 1);
   const int rcap = 2 * steps + 16;
-  pl.L_glb = make_layout(scope, wcap, (int)arena, smax + 2, rcap, 64, 0);
+  pl.L_glb = make_layout(scope, wcap, (int)arena, smax + 2, rcap, 64, 0, true);
   pl.glb_stride = pl.L_glb.total;
   const int64_t bytes_per_wave = pl.glb_stride * 4;
   const int64_t budget = (int64_t)16 << 30;

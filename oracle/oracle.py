@@ -77,7 +77,7 @@ def _concat(pairs):
         pb, tb = p.encode(), t.encode()
         p_off.append(pos); p_len.append(len(pb)); chunks.append(pb); pos += len(pb)
         t_off.append(pos); t_len.append(len(tb)); chunks.append(tb); pos += len(tb)
-    buf = b"".join(chunks) + b"\0" * 64
+    buf = np.frombuffer(b"".join(chunks) + b"\0" * 64, np.uint8)
     return (buf, np.array(p_off, np.int64), np.array(p_len, np.int32),
             np.array(t_off, np.int64), np.array(t_len, np.int32))
 
@@ -100,12 +100,45 @@ class _Result:
     def cigar(self, i):
         return runs_to_cigar(self.cigar_runs(i))
 
+    def compact(self):
+        """(offsets[n+1], runs) with the runs of all pairs back to back."""
+        n = len(self.nruns)
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(np.maximum(self.nruns, 0), out=off[1:])
+        idx = np.repeat(self.runs_off[:-1] - off[:-1], np.maximum(self.nruns, 0)) + np.arange(off[-1])
+        return off, self.runs[idx]
+
 
 def _p(a, ct):
     return a.ctypes.data_as(ctypes.POINTER(ct))
 
 
-class Wfa2Ref:
+def _prep(pairs_or_buf):
+    if isinstance(pairs_or_buf, tuple) and len(pairs_or_buf) == 5:
+        buf, p_off, p_len, t_off, t_len = pairs_or_buf
+        return (np.ascontiguousarray(buf, np.uint8), np.ascontiguousarray(p_off, np.int64),
+                np.ascontiguousarray(p_len, np.int32), np.ascontiguousarray(t_off, np.int64),
+                np.ascontiguousarray(t_len, np.int32))
+    return _concat(pairs_or_buf)
+
+
+class _Engine:
+    SYMBOL = None
+
+    def align(self, pairs, penalties=(0, 4, 6, 2), score_only=False, threads=0, max_steps=0):
+        """pairs: [(pattern, text), ...] or (ascii_buf, p_off, p_len, t_off, t_len)."""
+        buf, p_off, p_len, t_off, t_len = _prep(pairs)
+        n = len(p_len)
+        res = _Result(n, (p_len.astype(np.int64) + t_len + 1))
+        el = ctypes.c_double(0.0)
+        m, x, o, e = penalties
+        self._call(n, buf, p_off, p_len, t_off, t_len, m, x, o, e, score_only, max_steps,
+                   threads, res, el)
+        res.elapsed = el.value
+        return res
+
+
+class Wfa2Ref(_Engine):
     """The vendored WFA2-lib, exact BiWFA settings (memory ultralow, heuristic none)."""
 
     VARIANTS = {"v3": "libwfa2ref.so", "scalar": "libwfa2ref_scalar.so",
@@ -116,23 +149,16 @@ class Wfa2Ref:
         if not os.path.exists(path):
             raise FileNotFoundError(path + " (build with: make -C oracle ref)")
         self.lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
-        f = self.lib.wfa2ref_align_batch
-        f.restype = ctypes.c_int
         self.path = path
 
     @staticmethod
     def available(variant="v3"):
         return os.path.exists(os.path.join(REF_DIR, Wfa2Ref.VARIANTS[variant]))
 
-    def align(self, pairs, penalties=(0, 4, 6, 2), score_only=False, threads=0,
-              max_steps=0):
-        n = len(pairs)
-        buf, p_off, p_len, t_off, t_len = _concat(pairs)
-        res = _Result(n, (p_len.astype(np.int64) + t_len + 1))
-        el = ctypes.c_double(0.0)
-        m, x, o, e = penalties
+    def _call(self, n, buf, p_off, p_len, t_off, t_len, m, x, o, e, score_only, max_steps,
+              threads, res, el):
         self.lib.wfa2ref_align_batch(
-            ctypes.c_int(n), ctypes.c_char_p(buf),
+            ctypes.c_int(n), buf.ctypes.data_as(ctypes.c_char_p),
             _p(p_off, ctypes.c_int64), _p(p_len, ctypes.c_int32),
             _p(t_off, ctypes.c_int64), _p(t_len, ctypes.c_int32),
             m, x, o, e, int(score_only), int(max_steps), int(threads),
@@ -140,15 +166,13 @@ class Wfa2Ref:
             _p(res.status, ctypes.c_int32), _p(res.runs, ctypes.c_uint32),
             _p(res.runs_off, ctypes.c_int64), _p(res.nruns, ctypes.c_int32),
             ctypes.byref(el))
-        res.elapsed = el.value
-        return res
 
 
-class Oracle:
+class Oracle(_Engine):
     """The C restatement of WFA2-lib's BiWFA (oracle/biwfa_oracle.c)."""
 
-    def __init__(self):
-        path = os.path.join(HERE, "liboracle.so")
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             raise FileNotFoundError(path + " (build with: make -C oracle)")
         self.lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
@@ -158,18 +182,17 @@ class Oracle:
     def available():
         return os.path.exists(os.path.join(HERE, "liboracle.so"))
 
-    def align(self, pairs, penalties=(0, 4, 6, 2), score_only=False, threads=0,
-              max_steps=0, counters=False):
-        n = len(pairs)
-        buf, p_off, p_len, t_off, t_len = _concat(pairs)
-        res = _Result(n, (p_len.astype(np.int64) + t_len + 1))
+    def align(self, pairs, penalties=(0, 4, 6, 2), score_only=False, threads=0, max_steps=0,
+              counters=False):
+        return super().align(pairs, penalties, score_only, threads, max_steps)
+
+    def _call(self, n, buf, p_off, p_len, t_off, t_len, m, x, o, e, score_only, max_steps,
+              threads, res, el):
         res.wf_score = res.raw_score
         res.cells = np.zeros(n, np.int64)
         res.words = np.zeros(n, np.int64)
-        el = ctypes.c_double(0.0)
-        m, x, o, e = penalties
         self.lib.bwo_align_batch(
-            ctypes.c_int(n), ctypes.c_char_p(buf),
+            ctypes.c_int(n), buf.ctypes.data_as(ctypes.c_char_p),
             _p(p_off, ctypes.c_int64), _p(p_len, ctypes.c_int32),
             _p(t_off, ctypes.c_int64), _p(t_len, ctypes.c_int32),
             m, x, o, e, int(score_only), int(max_steps), int(threads),
@@ -178,5 +201,3 @@ class Oracle:
             _p(res.runs_off, ctypes.c_int64), _p(res.nruns, ctypes.c_int32),
             _p(res.cells, ctypes.c_int64), _p(res.words, ctypes.c_int64),
             ctypes.byref(el))
-        res.elapsed = el.value
-        return res

@@ -221,10 +221,7 @@ def test_planted_tie_break_mistake_is_caught(utest_pairs, tmp_path):
     subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-I", os.path.join(ROOT, "oracle"),
                     "-o", str(lib), str(cfile)], check=True)
     from oracle.oracle import Oracle
-    o = Oracle.__new__(Oracle)
-    import ctypes
-    o.lib = ctypes.CDLL(str(lib), mode=ctypes.RTLD_LOCAL)
-    o.path = str(lib)
+    o = Oracle(path=str(lib))
     gold = golden_alg("")
     r = o.align(utest_pairs, (0, 4, 6, 2))
     diffs = sum(1 for i, (s, c) in enumerate(gold) if r.cigar(i) != c)
