@@ -427,11 +427,16 @@ class DeviceBatch:
         if not self._h:
             raise BiwfaError("bw_dbatch_upload: " + last_error())
 
-    def set_schedule(self, schedule, predicted_cost=None):
-        cost = None
-        if predicted_cost is not None:
-            cost = np.ascontiguousarray(predicted_cost, np.float32)
-        st = self.aligner._L.bw_set_schedule(self.aligner._h, self._h, int(schedule), _ptr(cost))
+    def set_schedule(self, schedule, predicted_score=None):
+        """Pair order (and starting workspace tier) of later align() calls.  The
+        learned schedule takes one predicted alignment score per pair; it never
+        changes any result, only where and when each pair runs."""
+        pred = None
+        if predicted_score is not None:
+            pred = np.ascontiguousarray(predicted_score, np.float32)
+            if len(pred) != self.n:
+                raise BiwfaError("one predicted score per pair is needed")
+        st = self.aligner._L.bw_set_schedule(self.aligner._h, self._h, int(schedule), _ptr(pred))
         if st != 0:
             raise BiwfaError("bw_set_schedule: " + last_error())
 

@@ -50,7 +50,7 @@ def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     hipcc = _hipcc()
     objs = []
-    for src in ("biwfa_kernel.hip", "runtime.cpp"):
+    for src in ("biwfa_kernel.hip", "runtime.cpp", "costmodel.hip"):
         obj = os.path.join(OBJ, src.split(".")[0] + ".o")
         _run([hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)

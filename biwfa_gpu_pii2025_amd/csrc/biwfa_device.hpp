@@ -50,19 +50,20 @@ struct KParams {
   const int32_t* p_len;
   const int64_t* t_woff;
   const int32_t* t_len;
-  // work source: order[work_list ? work_list[i] : i] for i < count
-  const int32_t* order;
-  const int32_t* work_list;      // optional indirection (spill list)
-  const int32_t* work_count;     // device count (if non-null) else num_work
-  int32_t num_work;
-  int32_t* queue;                // [0] = next work index (atomic)
+  // work source of this tier: pair ids list[0 .. *count), taken through `queue`
+  const int32_t* list;
+  const int32_t* count;          // device count (the host's initial part plus the
+                                 // previous tier's spills)
+  int32_t* queue;                // next list index (atomic)
   int32_t chunk;
-  // spill output (LDS kernel)
-  int32_t* spill_count;
-  int32_t* spill_list;
+  // next tier: pairs that outgrow this tier's capacities are appended here
+  // (nullptr: last tier, such a pair ends with kStCapacity)
+  int32_t* next_list;
+  int32_t* next_count;
   // penalties: internal (after the match<0 transform) and user (for the .alg score)
   int32_t x, o, e, scope;
   int32_t um, ux, uo, ue;
+  int32_t gmask;                 // g-1 when every penalty is a multiple of g = 2^j, else 0
   int32_t score_only;
   int32_t max_steps;
   // outputs (device, indexed by pair id)

@@ -117,6 +117,9 @@ struct Ctx {
   // headers in registers: ring (lane = ring*scope + slot) and base case (lane = score)
   int rlo[3], rhi[3];
   int blo[3], bhi[3], bkb[3];
+  // presence masks (scalar): ring bit ring*32+slot, base-case bit score (LDS kernel)
+  uint64_t rpres[3], bpres[3];
+  int gmask;      // scores s with (s & gmask) != 0 are null steps (even penalties)
   uint32_t* out;
   int out_cap, out_n, pend_op, pend_len, alg_score;
   int spill;      // a workspace capacity was exceeded
@@ -133,16 +136,15 @@ DI int score_guard(const Ctx<kLds>& c, int plen, int tlen) {
   return mn * c.x + c.o + df * c.e + c.o + 2 * c.scope + 16;
 }
 
-// Register-array access with a run-time component index.  The values pass
-// through an empty asm so that the selection stays a selection of values: a
-// select of the three addresses would pin the aligner state to scratch memory.
-DI int opaque(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
+// Register-array access with a run-time component index, written as masking
+// arithmetic rather than a select: a select of the three elements would be
+// folded into a select of their addresses and pin the aligner state to scratch.
 DI int sel3(const int (&a)[3], int comp) {
-  const int a0 = opaque(a[0]), a1 = opaque(a[1]), a2 = opaque(a[2]);
-  return comp == CM ? a0 : (comp == CI ? a1 : a2);
+  return (a[0] & -(int)(comp == CM)) | (a[1] & -(int)(comp == CI)) | (a[2] & -(int)(comp == CD));
+}
+DI uint64_t sel3u(const uint64_t (&a)[3], int comp) {
+  return (a[0] & -(uint64_t)(comp == CM)) | (a[1] & -(uint64_t)(comp == CI)) |
+         (a[2] & -(uint64_t)(comp == CD));
 }
 
 // ring slot of (score - delta) given the slot of score; 0 <= delta < scope
@@ -159,6 +161,14 @@ DI int ring_base(const Ctx<kLds>& c, int ring, int slot, int comp) {
   return c.L.data + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
 }
 
+// Presence bit of (ring slot | base score) for component `comp`: ring slots use
+// bit ring*32 + slot, base-case scores (LDS kernel) bit score.
+template <bool kLds>
+DI bool is_present(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp) {
+  if (d.ring >= 0) return (sel3u(c.rpres, comp) >> (d.ring * 32 + slot)) & 1ull;
+  return (sel3u(c.bpres, comp) >> score) & 1ull;
+}
+
 // Component `comp` at `score` (ring slot `slot`), WFA2's
 // wavefront_compute_get_*wavefront (wavefront_compute.c:266-305).  *present
 // reports a non-NULL WFA2 pointer (possibly with lo > hi).
@@ -167,16 +177,17 @@ DI Wf get_wf(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, in
   *present = 0;
   if (score < 0) return wf_null();
   if (d.ring >= 0) {
+    if (!is_present(c, d, score, slot, comp)) return wf_null();
+    *present = 1;
     const int j = d.ring * c.scope + slot;
     const int lo = rdl(sel3(c.rlo, comp), j);
-    if (lo == kAbsent) return wf_null();
-    *present = 1;
     const int hi = rdl(sel3(c.rhi, comp), j);
     if (lo > hi) return Wf{lo, hi, 0, 0};
     return Wf{lo, hi, ring_base(c, d.ring, slot, comp), c.L.wcap - 1};
   }
   int lo, hi, kb;
   if (kLds) {
+    if (!is_present(c, d, score, slot, comp)) return wf_null();
     lo = rdl(sel3(c.blo, comp), score);
     hi = rdl(sel3(c.bhi, comp), score);
     kb = rdl(sel3(c.bkb, comp), score);
@@ -201,23 +212,37 @@ DI Wf get_in(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp) {
 
 DI void lane_set(int (&a)[3], int comp, int lane, int j, int v) {
   const bool me = lane == j;
-  const int a0 = opaque(a[0]), a1 = opaque(a[1]), a2 = opaque(a[2]);
-  a[0] = (me && comp == CM) ? v : a0;
-  a[1] = (me && comp == CI) ? v : a1;
-  a[2] = (me && comp == CD) ? v : a2;
+  a[0] = (me && comp == CM) ? v : a[0];
+  a[1] = (me && comp == CI) ? v : a[1];
+  a[2] = (me && comp == CD) ? v : a[2];
+}
+
+DI void bit_set(uint64_t (&m)[3], int comp, int bit, bool on) {
+  const uint64_t b = 1ull << bit;
+  const uint64_t set = on ? b : 0ull;
+  m[0] = comp == CM ? ((m[0] & ~b) | set) : m[0];
+  m[1] = comp == CI ? ((m[1] & ~b) | set) : m[1];
+  m[2] = comp == CD ? ((m[2] & ~b) | set) : m[2];
 }
 
 // lo == kAbsent marks an unallocated component
 template <bool kLds>
 DI void set_hdr(Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, int lo, int hi, int kb) {
+  const bool on = lo != kAbsent;
   if (d.ring >= 0) {
-    const int j = d.ring * c.scope + slot;
-    lane_set(c.rlo, comp, c.lane, j, lo);
-    lane_set(c.rhi, comp, c.lane, j, hi);
+    bit_set(c.rpres, comp, d.ring * 32 + slot, on);
+    if (on) {
+      const int j = d.ring * c.scope + slot;
+      lane_set(c.rlo, comp, c.lane, j, lo);
+      lane_set(c.rhi, comp, c.lane, j, hi);
+    }
   } else if (kLds) {
-    lane_set(c.blo, comp, c.lane, score, lo);
-    lane_set(c.bhi, comp, c.lane, score, hi);
-    lane_set(c.bkb, comp, c.lane, score, kb);
+    bit_set(c.bpres, comp, score, on);
+    if (on) {
+      lane_set(c.blo, comp, c.lane, score, lo);
+      lane_set(c.bhi, comp, c.lane, score, hi);
+      lane_set(c.bkb, comp, c.lane, score, kb);
+    }
   } else {
     const int h = c.L.bhdr + (score * 3 + comp) * 3;
     c.ws[h] = lo;
@@ -266,19 +291,27 @@ DI int lcp(const Ctx<kLds>& c, const Dir& d, int v, int h) {
 // `slot` is score % scope for ring aligners.
 
 template <bool kLds>
+DI void null_step(Ctx<kLds>& c, Dir& d, int s, int slot) {
+  d.null_steps++;
+  set_hdr(c, d, s, slot, CM, kAbsent, 0, 0);
+  set_hdr(c, d, s, slot, CI, kAbsent, 0, 0);
+  set_hdr(c, d, s, slot, CD, kAbsent, 0, 0);
+}
+
+template <bool kLds>
 DI void compute_step(Ctx<kLds>& c, Dir& d, int s, int slot) {
   const int sc = c.scope;
+  if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return; }
+  // With penalties that share a power-of-two factor g, only multiples of g are
+  // reachable scores: the other steps are null without looking at the inputs.
+  if (s & c.gmask) { null_step(c, d, s, slot); return; }
   const Wf mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
   const Wf mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
   const Wf ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
   const Wf de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
-  if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return; }
   const bool nmm = mm.lo > mm.hi, nmo = mo.lo > mo.hi, nie = ie.lo > ie.hi, nde = de.lo > de.hi;
-  if (nmm && nmo && nie && nde) {               // null step
-    d.null_steps++;
-    set_hdr(c, d, s, slot, CM, kAbsent, 0, 0);
-    set_hdr(c, d, s, slot, CI, kAbsent, 0, 0);
-    set_hdr(c, d, s, slot, CD, kAbsent, 0, 0);
+  if (nmm && nmo && nie && nde) {               // null step (wavefront_compute_affine.c:245-251)
+    null_step(c, d, s, slot);
     return;
   }
   d.null_steps = 0;
@@ -463,8 +496,8 @@ DI void overlap(const Ctx<kLds>& c, const Dir& a0, const Dir& a1, int s0, int sl
 // wavefront_aligner_init_wf (wavefront_aligner.c:311-383) for a ring aligner.
 template <bool kLds>
 DI void ring_init(Ctx<kLds>& c, Dir& d) {
-  const bool mine = c.lane >= d.ring * c.scope && c.lane < (d.ring + 1) * c.scope;
-  for (int q = 0; q < 3; ++q) c.rlo[q] = mine ? kAbsent : c.rlo[q];
+  const uint64_t keep = ~(0xffffffffull << (32 * d.ring));
+  for (int q = 0; q < 3; ++q) c.rpres[q] &= keep;
   set_hdr(c, d, 0, 0, d.cb, 0, 0, 0);
   c.ws[ring_base(c, d.ring, 0, d.cb)] = 0;
   d.null_steps = 0;
@@ -800,7 +833,7 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
   const int waves_per_block = blockDim.x / kWave;
   int32_t* ws = kLds ? lds + wv * p.L.total
                      : p.gws + (int64_t)(blockIdx.x * waves_per_block + wv) * p.gws_stride;
-  const int count = p.work_count ? uni(*(volatile const int32_t*)p.work_count) : p.num_work;
+  const int count = uni(*(volatile const int32_t*)p.count);
   for (;;) {
     int base = 0;
     if (lane == 0) base = atomicAdd(p.queue, p.chunk);
@@ -808,8 +841,7 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
     if (base >= count) break;
     const int end = imin(base + p.chunk, count);
     for (int w = base; w < end; ++w) {
-      const int idx = p.work_list ? uni(p.work_list[w]) : w;
-      const int pair = p.order ? uni(p.order[idx]) : idx;
+      const int pair = uni(p.list[w]);
       const int plen = uni(p.p_len[pair]), tlen = uni(p.t_len[pair]);
       const uint64_t* gP = p.words + p.p_woff[pair];
       const uint64_t* gT = p.words + p.t_woff[pair];
@@ -824,7 +856,9 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
       for (int q = 0; q < 3; ++q) {
         c.rlo[q] = kAbsent; c.rhi[q] = 0;
         c.blo[q] = kAbsent; c.bhi[q] = 0; c.bkb[q] = 0;
+        c.rpres[q] = 0; c.bpres[q] = 0;
       }
+      c.gmask = p.gmask;
       c.out = p.runs + p.runs_off[pair];
       c.out_cap = (int)(p.runs_off[pair + 1] - p.runs_off[pair]);
       c.out_n = 0; c.pend_op = -1; c.pend_len = 0; c.alg_score = 0;
@@ -855,10 +889,10 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
       int wf = INT_MIN, st = kStCapacity;
       if (!c.spill) st = align_pair(c, plen, tlen, p.score_only, &wf);
       if (c.spill) {
-        if (kLds) {
+        if (p.next_list) {      // re-run by the next tier with larger capacities
           if (lane == 0) {
-            const int slot = atomicAdd(p.spill_count, 1);
-            p.spill_list[slot] = pair;
+            const int slot = atomicAdd(p.next_count, 1);
+            p.next_list[slot] = pair;
           }
           continue;
         }
@@ -875,11 +909,11 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
   }
 }
 
-// Resets the queue/spill counters (a kernel rather than a memset so that the
-// launch sequence can be captured in a graph).
-__global__ void biwfa_reset(int32_t* counters, int n) {
+// Sets every tier's {queue, count} counters to their initial values (a kernel
+// rather than a memcpy/memset so that the launch sequence can be captured in a graph).
+__global__ void biwfa_reset(int32_t* counters, const int32_t* init, int n) {
   const int i = threadIdx.x;
-  if (i < n) counters[i] = 0;
+  if (i < n) counters[i] = init[i];
 }
 
 // Copies each pair's runs into the compacted layout (one wave per pair).

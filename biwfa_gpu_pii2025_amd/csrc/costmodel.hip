@@ -1,0 +1,479 @@
+// costmodel.hip -- learned pair-cost model for the BiWFA scheduler (gfx950).
+//
+// A 4 -> 64 -> 64 -> 1 ReLU MLP predicts a pair's alignment score from
+//   x0 = log2(p+1)/16, x1 = log2(t+1)/16, x2 = log2(|p-t|+1)/16,
+//   x3 = fraction of 16 pattern 12-mers (evenly spaced) found in the text near
+//        their expected position,
+// target y = log2(1 + score)/16.  Weights are stored as BF16 (round to nearest
+// even of the FP32 master copy) and every product is accumulated in FP32.  The
+// prediction only orders pairs and picks their starting workspace tier; it
+// never changes an alignment result.
+//
+// Kernels: feature extraction (one wave per pair over the 2-bit packed batch),
+// batched forward, per-block backward partial sums reduced in a fixed order
+// (deterministic gradients), and an Adam step.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "biwfa_mi355x.h"
+
+namespace bwcm {
+
+constexpr int kIn = 4, kH = 64;
+// parameter vector layout (floats)
+constexpr int kW1 = 0;                 // [kH][kIn]
+constexpr int kB1 = kW1 + kH * kIn;    // [kH]
+constexpr int kW2 = kB1 + kH;          // [kH][kH]  (row = output unit)
+constexpr int kB2 = kW2 + kH * kH;     // [kH]
+constexpr int kW3 = kB2 + kH;          // [kH]
+constexpr int kB3 = kW3 + kH;          // [1]
+constexpr int kParams = kB3 + 1;
+constexpr int kSamplesPerBlock = 64;   // backward block tile
+
+// FP32 -> BF16 (round to nearest, ties to even) -> FP32
+__host__ __device__ inline float bf16_round(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return f;   // inf / nan unchanged
+  const uint32_t lsb = (u >> 16) & 1u;
+  u = (u + 0x7fffu + lsb) & 0xffff0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// features
+
+__device__ inline uint64_t window(const uint64_t* s, int pos) {   // bases [pos, pos+32)
+  const int wi = pos >> 5;
+  const int sh = (pos & 31) << 1;
+  const uint64_t lo = s[wi], hi = s[wi + 1];
+  return (lo >> sh) | ((hi << 1) << (63 - sh));
+}
+
+__global__ __launch_bounds__(256) void features_kernel(const uint64_t* words, const int64_t* p_woff,
+                                                       const int32_t* p_len, const int64_t* t_woff,
+                                                       const int32_t* t_len, int n, float* out) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= n) return;
+  const int p = p_len[wave], t = t_len[wave];
+  const uint64_t* P = words + p_woff[wave];
+  const uint64_t* T = words + t_woff[wave];
+  const int k = 12;
+  const uint64_t kmask = (1ull << (2 * k)) - 1;
+  int found = 0;
+  if (p >= k && t >= k) {
+    const int span = abs(p - t);
+    const int W = 96 + span;
+    for (int i = 0; i < 16; ++i) {
+      const int pos = (int)(((long long)(2 * i + 1) * (p - k)) / 32);
+      const uint64_t key = window(P, pos) & kmask;
+      const int q = (int)(((long long)pos * (t - k)) / (p - k > 0 ? p - k : 1));
+      const int lo = max(0, q - W), hi = min(t - k, q + W);
+      bool hit = false;
+      for (int j0 = lo; j0 <= hi && !hit; j0 += 64) {
+        const int j = j0 + lane;
+        const bool m = (j <= hi) && ((window(T, j) & kmask) == key);
+        hit = __ballot(m) != 0ull;
+      }
+      found += hit ? 1 : 0;
+    }
+  }
+  if (lane == 0) {
+    float* o = out + (size_t)wave * kIn;
+    o[0] = log2f((float)p + 1.0f) / 16.0f;
+    o[1] = log2f((float)t + 1.0f) / 16.0f;
+    o[2] = log2f((float)abs(p - t) + 1.0f) / 16.0f;
+    o[3] = (p >= k && t >= k) ? (float)found / 16.0f : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MLP
+
+// Copies the FP32 master parameters rounded to BF16 (the stored weights).
+__global__ void to_bf16_kernel(const float* master, float* w, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = bf16_round(master[i]);
+}
+
+// One sample through the network (weights in LDS, activations in registers:
+// every loop is fully unrolled so that h1/h2 never leave the register file).
+__device__ inline void forward_one(const float* w, const float* x, float* h1, float* h2, float* y) {
+#pragma unroll
+  for (int j = 0; j < kH; ++j) {
+    float a = w[kB1 + j];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) a = fmaf(w[kW1 + j * kIn + i], x[i], a);
+    h1[j] = a > 0.f ? a : 0.f;
+  }
+#pragma unroll
+  for (int j = 0; j < kH; ++j) {
+    float a = w[kB2 + j];
+#pragma unroll
+    for (int i = 0; i < kH; ++i) a = fmaf(w[kW2 + j * kH + i], h1[i], a);
+    h2[j] = a > 0.f ? a : 0.f;
+  }
+  float o = w[kB3];
+#pragma unroll
+  for (int i = 0; i < kH; ++i) o = fmaf(w[kW3 + i], h2[i], o);
+  *y = o;
+}
+
+__global__ __launch_bounds__(256) void forward_kernel(const float* w_bf16, const float* X, int n, float* Y) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = w_bf16[i];
+  __syncthreads();
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  float h1[kH], h2[kH], y;
+  forward_one(w, X + (size_t)s * kIn, h1, h2, &y);
+  Y[s] = y;
+}
+
+// Gradient of the mean squared error over samples [0,n) w.r.t. the (BF16)
+// weights; block b writes its partial sum over samples [64b, 64b+64).
+__global__ __launch_bounds__(256) void backward_kernel(const float* w_bf16, const float* X, const float* Yt,
+                                                       int n, float inv_n, float* partial) {
+  __shared__ float w[kParams];
+  __shared__ float sx[kSamplesPerBlock][kIn];
+  __shared__ float sh1[kSamplesPerBlock][kH];
+  __shared__ float sd2[kSamplesPerBlock][kH];   // dL/d(pre-activation of layer 2)
+  __shared__ float sd1[kSamplesPerBlock][kH];   // dL/d(pre-activation of layer 1)
+  __shared__ float sg3[kSamplesPerBlock];       // dL/dy
+  __shared__ float sh2[kSamplesPerBlock][kH];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = w_bf16[i];
+  __syncthreads();
+  const int base = blockIdx.x * kSamplesPerBlock;
+  const int tid = threadIdx.x;
+  if (tid < kSamplesPerBlock) {
+    const int s = base + tid;
+    float h1[kH], h2[kH], y = 0.f;
+    float x[kIn] = {0.f, 0.f, 0.f, 0.f};
+    if (s < n) {
+#pragma unroll
+      for (int i = 0; i < kIn; ++i) x[i] = X[(size_t)s * kIn + i];
+      forward_one(w, x, h1, h2, &y);
+    } else {
+#pragma unroll
+      for (int i = 0; i < kH; ++i) { h1[i] = 0.f; h2[i] = 0.f; }
+    }
+    const float g = (s < n) ? 2.0f * (y - Yt[s]) * inv_n : 0.0f;
+    sg3[tid] = g;
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) sx[tid][i] = x[i];
+#pragma unroll
+    for (int j = 0; j < kH; ++j) {
+      sh1[tid][j] = h1[j];
+      sh2[tid][j] = h2[j];
+      sd2[tid][j] = (h2[j] > 0.f) ? g * w[kW3 + j] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < kH; ++i) {
+      float a = 0.f;
+      for (int j = 0; j < kH; ++j) a = fmaf(w[kW2 + j * kH + i], sd2[tid][j], a);
+      sd1[tid][i] = (h1[i] > 0.f) ? a : 0.f;
+    }
+  }
+  __syncthreads();
+  float* out = partial + (size_t)blockIdx.x * kParams;
+  // every parameter's partial sum over the block's samples, in sample order
+  for (int q = tid; q < kParams; q += blockDim.x) {
+    float acc = 0.f;
+    if (q < kB1) {
+      const int j = (q - kW1) / kIn, i = (q - kW1) % kIn;
+      for (int s = 0; s < kSamplesPerBlock; ++s) acc = fmaf(sd1[s][j], sx[s][i], acc);
+    } else if (q < kW2) {
+      const int j = q - kB1;
+      for (int s = 0; s < kSamplesPerBlock; ++s) acc += sd1[s][j];
+    } else if (q < kB2) {
+      const int j = (q - kW2) / kH, i = (q - kW2) % kH;
+      for (int s = 0; s < kSamplesPerBlock; ++s) acc = fmaf(sd2[s][j], sh1[s][i], acc);
+    } else if (q < kW3) {
+      const int j = q - kB2;
+      for (int s = 0; s < kSamplesPerBlock; ++s) acc += sd2[s][j];
+    } else if (q < kB3) {
+      const int i = q - kW3;
+      for (int s = 0; s < kSamplesPerBlock; ++s) acc = fmaf(sg3[s], sh2[s][i], acc);
+    } else {
+      for (int s = 0; s < kSamplesPerBlock; ++s) acc += sg3[s];
+    }
+    out[q] = acc;
+  }
+}
+
+// grad[q] = sum over blocks of partial[b][q] (fixed order: deterministic)
+__global__ void reduce_kernel(const float* partial, int blocks, float* grad) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= kParams) return;
+  float acc = 0.f;
+  for (int b = 0; b < blocks; ++b) acc += partial[(size_t)b * kParams + q];
+  grad[q] = acc;
+}
+
+__global__ void adam_kernel(float* master, const float* grad, float* m, float* v, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= kParams) return;
+  const float g = grad[q];
+  const float mq = b1 * m[q] + (1.f - b1) * g;
+  const float vq = b2 * v[q] + (1.f - b2) * g * g;
+  m[q] = mq;
+  v[q] = vq;
+  master[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+}
+
+}  // namespace bwcm
+
+using namespace bwcm;
+
+struct bw_mlp {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  float* d_master = nullptr;
+  float* d_w = nullptr;       // BF16-rounded weights used by forward/backward
+  float* d_m = nullptr;
+  float* d_v = nullptr;
+  float* d_grad = nullptr;
+  int64_t steps = 0;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(const char* what, hipError_t e) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return BW_STATUS_HIP_ERROR;
+}
+
+#define CM_TRY(expr)                                   \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return fail(#expr, _e);      \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t n) { return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (n ? n : 1)); }
+};
+
+int refresh_bf16(bw_mlp* m) {
+  hipLaunchKernelGGL(to_bf16_kernel, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, m->d_master,
+                     m->d_w, kParams);
+  CM_TRY(hipGetLastError());
+  return 0;
+}
+
+// He-style initialisation from a seeded SplitMix64 stream (deterministic).
+void init_params(std::vector<float>& w, uint64_t seed) {
+  uint64_t x = seed * 0x9E3779B97F4A7C15ull + 12345;
+  auto next = [&]() {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto uni = [&](double a) { return (float)((2.0 * next() - 1.0) * a); };
+  w.assign(kParams, 0.f);
+  for (int i = 0; i < kH * kIn; ++i) w[kW1 + i] = uni(std::sqrt(6.0 / kIn));
+  for (int i = 0; i < kH; ++i) w[kB1 + i] = 0.05f;
+  for (int i = 0; i < kH * kH; ++i) w[kW2 + i] = uni(std::sqrt(6.0 / kH));
+  for (int i = 0; i < kH; ++i) w[kB2 + i] = 0.05f;
+  for (int i = 0; i < kH; ++i) w[kW3 + i] = uni(std::sqrt(3.0 / kH));
+  w[kB3] = 0.f;
+}
+}  // namespace
+
+extern "C" {
+
+int bw_mlp_num_params(void) { return kParams; }
+
+bw_mlp_t* bw_mlp_new(int device, uint64_t seed) {
+  bw_mlp* m = new bw_mlp();
+  m->dev = device;
+  std::vector<float> w;
+  init_params(w, seed);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&m->d_master), 4 * kParams) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&m->d_w), 4 * kParams) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&m->d_m), 4 * kParams) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&m->d_v), 4 * kParams) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&m->d_grad), 4 * kParams) != hipSuccess ||
+      hipMemcpy(m->d_master, w.data(), 4 * kParams, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(m->d_m, 0, 4 * kParams) != hipSuccess || hipMemset(m->d_v, 0, 4 * kParams) != hipSuccess ||
+      refresh_bf16(m) != 0 || hipStreamSynchronize(m->stream) != hipSuccess) {
+    g_err = "bw_mlp_new: device setup failed";
+    bw_mlp_free(m);
+    return nullptr;
+  }
+  return m;
+}
+
+void bw_mlp_free(bw_mlp_t* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->dev);
+  for (float* p : {m->d_master, m->d_w, m->d_m, m->d_v, m->d_grad})
+    if (p) (void)hipFree(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+const char* bw_mlp_last_error(void) { return g_err.c_str(); }
+
+int bw_mlp_get_params(bw_mlp_t* m, float* master, float* bf16_weights) {
+  if (!m) return BW_STATUS_INVALID_ARG;
+  CM_TRY(hipSetDevice(m->dev));
+  if (master) CM_TRY(hipMemcpy(master, m->d_master, 4 * kParams, hipMemcpyDeviceToHost));
+  if (bf16_weights) CM_TRY(hipMemcpy(bf16_weights, m->d_w, 4 * kParams, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int bw_mlp_set_params(bw_mlp_t* m, const float* master) {
+  if (!m || !master) return BW_STATUS_INVALID_ARG;
+  CM_TRY(hipSetDevice(m->dev));
+  CM_TRY(hipMemcpy(m->d_master, master, 4 * kParams, hipMemcpyHostToDevice));
+  const int st = refresh_bf16(m);
+  if (st) return st;
+  CM_TRY(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+// Predictions for n samples (host X[n][4] -> host y[n]).
+int bw_mlp_predict(bw_mlp_t* m, const float* X, int64_t n, float* y) {
+  if (!m || n < 0 || (n > 0 && (!X || !y))) return BW_STATUS_INVALID_ARG;
+  if (n == 0) return 0;
+  CM_TRY(hipSetDevice(m->dev));
+  DevBuf<float> dX, dY;
+  CM_TRY(dX.alloc(n * kIn));
+  CM_TRY(dY.alloc(n));
+  CM_TRY(hipMemcpyAsync(dX.p, X, 4 * n * kIn, hipMemcpyHostToDevice, m->stream));
+  hipLaunchKernelGGL(forward_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->d_w, dX.p,
+                     (int)n, dY.p);
+  CM_TRY(hipGetLastError());
+  CM_TRY(hipMemcpyAsync(y, dY.p, 4 * n, hipMemcpyDeviceToHost, m->stream));
+  CM_TRY(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+// Gradient of the mean squared error over all n samples (host), for checking.
+int bw_mlp_gradient(bw_mlp_t* m, const float* X, const float* y, int64_t n, float* grad) {
+  if (!m || n <= 0 || !X || !y || !grad) return BW_STATUS_INVALID_ARG;
+  CM_TRY(hipSetDevice(m->dev));
+  const int blocks = (int)((n + kSamplesPerBlock - 1) / kSamplesPerBlock);
+  DevBuf<float> dX, dY, dP;
+  CM_TRY(dX.alloc(n * kIn));
+  CM_TRY(dY.alloc(n));
+  CM_TRY(dP.alloc((size_t)blocks * kParams));
+  CM_TRY(hipMemcpyAsync(dX.p, X, 4 * n * kIn, hipMemcpyHostToDevice, m->stream));
+  CM_TRY(hipMemcpyAsync(dY.p, y, 4 * n, hipMemcpyHostToDevice, m->stream));
+  hipLaunchKernelGGL(backward_kernel, dim3(blocks), dim3(256), 0, m->stream, m->d_w, dX.p, dY.p, (int)n,
+                     1.0f / (float)n, dP.p);
+  hipLaunchKernelGGL(reduce_kernel, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, dP.p, blocks,
+                     m->d_grad);
+  CM_TRY(hipGetLastError());
+  CM_TRY(hipMemcpyAsync(grad, m->d_grad, 4 * kParams, hipMemcpyDeviceToHost, m->stream));
+  CM_TRY(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+// Mini-batch Adam training on the device.  Samples are visited in a seeded
+// permutation per epoch; *final_loss receives the full-set MSE after training.
+int bw_mlp_train(bw_mlp_t* m, const float* X, const float* y, int64_t n, int epochs, int batch, float lr,
+                 uint64_t seed, float* final_loss) {
+  if (!m || n <= 0 || !X || !y || epochs < 0 || batch <= 0) return BW_STATUS_INVALID_ARG;
+  CM_TRY(hipSetDevice(m->dev));
+  DevBuf<float> dX, dY, dXb, dYb, dP, dPred;
+  CM_TRY(dX.alloc(n * kIn));
+  CM_TRY(dY.alloc(n));
+  CM_TRY(dXb.alloc((size_t)batch * kIn));
+  CM_TRY(dYb.alloc(batch));
+  const int max_blocks = (batch + kSamplesPerBlock - 1) / kSamplesPerBlock;
+  CM_TRY(dP.alloc((size_t)max_blocks * kParams));
+  CM_TRY(dPred.alloc(n));
+  // host-side shuffled copies per epoch (the permutation is cheap; the math is on the device)
+  std::vector<float> hx((size_t)batch * kIn), hy(batch);
+  std::vector<int64_t> perm(n);
+  for (int64_t i = 0; i < n; ++i) perm[i] = i;
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 7;
+  auto rnd = [&]() {
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    return st;
+  };
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (int64_t i = n - 1; i > 0; --i) std::swap(perm[i], perm[rnd() % (uint64_t)(i + 1)]);
+    for (int64_t b0 = 0; b0 < n; b0 += batch) {
+      const int nb = (int)std::min<int64_t>(batch, n - b0);
+      for (int j = 0; j < nb; ++j) {
+        const int64_t s = perm[b0 + j];
+        for (int q = 0; q < kIn; ++q) hx[(size_t)j * kIn + q] = X[s * kIn + q];
+        hy[j] = y[s];
+      }
+      CM_TRY(hipMemcpyAsync(dXb.p, hx.data(), 4 * (size_t)nb * kIn, hipMemcpyHostToDevice, m->stream));
+      CM_TRY(hipMemcpyAsync(dYb.p, hy.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, m->stream));
+      const int blocks = (nb + kSamplesPerBlock - 1) / kSamplesPerBlock;
+      hipLaunchKernelGGL(backward_kernel, dim3(blocks), dim3(256), 0, m->stream, m->d_w, dXb.p, dYb.p, nb,
+                         1.0f / (float)nb, dP.p);
+      hipLaunchKernelGGL(reduce_kernel, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, dP.p, blocks,
+                         m->d_grad);
+      m->steps++;
+      const float c1 = 1.f - std::pow(b1, (float)m->steps), c2 = 1.f - std::pow(b2, (float)m->steps);
+      hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, m->d_master,
+                         m->d_grad, m->d_m, m->d_v, lr, b1, b2, eps, c1, c2);
+      const int r = refresh_bf16(m);
+      if (r) return r;
+      CM_TRY(hipGetLastError());
+      CM_TRY(hipStreamSynchronize(m->stream));   // the host buffers are reused next batch
+    }
+  }
+  if (final_loss) {
+    std::vector<float> pred(n);
+    CM_TRY(hipMemcpyAsync(dX.p, X, 4 * n * kIn, hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(forward_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->d_w,
+                       dX.p, (int)n, dPred.p);
+    CM_TRY(hipGetLastError());
+    CM_TRY(hipMemcpyAsync(pred.data(), dPred.p, 4 * n, hipMemcpyDeviceToHost, m->stream));
+    CM_TRY(hipStreamSynchronize(m->stream));
+    double acc = 0.0;
+    for (int64_t i = 0; i < n; ++i) acc += (double)(pred[i] - y[i]) * (pred[i] - y[i]);
+    *final_loss = (float)(acc / (double)n);
+  }
+  return 0;
+}
+
+// Features of every pair of a packed host batch (X[n][4], host).
+int bw_batch_features(int device, const bw_batch_t* b, float* X) {
+  if (!b || b->num_pairs < 0 || (b->num_pairs > 0 && !X)) return BW_STATUS_INVALID_ARG;
+  const int64_t n = b->num_pairs;
+  if (n == 0) return 0;
+  CM_TRY(hipSetDevice(device));
+  DevBuf<uint64_t> dW;
+  DevBuf<int64_t> dPo, dTo;
+  DevBuf<int32_t> dPl, dTl;
+  DevBuf<float> dX;
+  CM_TRY(dW.alloc(b->num_words));
+  CM_TRY(dPo.alloc(n)); CM_TRY(dTo.alloc(n)); CM_TRY(dPl.alloc(n)); CM_TRY(dTl.alloc(n));
+  CM_TRY(dX.alloc(n * kIn));
+  CM_TRY(hipMemcpy(dW.p, b->words, 8 * b->num_words, hipMemcpyHostToDevice));
+  CM_TRY(hipMemcpy(dPo.p, b->pattern_word_offset, 8 * n, hipMemcpyHostToDevice));
+  CM_TRY(hipMemcpy(dTo.p, b->text_word_offset, 8 * n, hipMemcpyHostToDevice));
+  CM_TRY(hipMemcpy(dPl.p, b->pattern_length, 4 * n, hipMemcpyHostToDevice));
+  CM_TRY(hipMemcpy(dTl.p, b->text_length, 4 * n, hipMemcpyHostToDevice));
+  const int waves_per_block = 4;
+  hipLaunchKernelGGL(features_kernel, dim3((unsigned)((n + waves_per_block - 1) / waves_per_block)),
+                     dim3(64 * waves_per_block), 0, 0, dW.p, dPo.p, dPl.p, dTo.p, dTl.p, (int)n, dX.p);
+  CM_TRY(hipGetLastError());
+  CM_TRY(hipMemcpy(X, dX.p, 4 * n * kIn, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

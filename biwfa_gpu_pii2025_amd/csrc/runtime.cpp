@@ -23,7 +23,7 @@
 namespace bw {
 template <bool kLds>
 __global__ void biwfa_persistent(KParams p);
-__global__ void biwfa_reset(int32_t* counters, int n);
+__global__ void biwfa_reset(int32_t* counters, const int32_t* init, int n);
 __global__ void biwfa_compact(const uint32_t* runs, const int64_t* runs_off, const int32_t* nruns,
                               const int64_t* out_off, uint32_t* out, int n);
 }  // namespace bw
@@ -123,6 +123,21 @@ int internal_penalties(const bw_affine_penalties_t& p, Penalties* out) {
 
 }  // namespace
 
+// One launch of the persistent kernel over a list of pairs with fixed capacities.
+// Tier 0 keeps its workspace in LDS; later tiers use per-wave global slabs whose
+// ring width grows geometrically.  A pair that outgrows its tier (ring width,
+// base-case arena, headers, recursion stack or LDS staging) is appended to the
+// next tier's list on the device; the last tier holds the batch's worst case.
+struct Tier {
+  bool lds = false;
+  bw::WsLayout L{};
+  int waves_per_block = bw::kWavesPerBlock;
+  int blocks = 0;
+  int64_t stride = 0;        // global tiers: ints per wave
+  int chunk = 1;
+  int seq_cap = 0;           // LDS tier: longest sequence it can stage
+};
+
 struct bw_dbatch {
   int64_t n = 0, nwords = 0;
   uint64_t* d_words = nullptr;
@@ -132,8 +147,9 @@ struct bw_dbatch {
   int32_t* d_tlen = nullptr;
   std::vector<int32_t> h_plen, h_tlen;
   int max_p = 0, max_t = 0, max_pt = 0;
-  int32_t* d_order = nullptr;
-  int32_t* d_spill = nullptr;
+  std::vector<Tier> tiers;
+  int32_t* d_lists = nullptr;   // tiers x n pair ids
+  int32_t* d_init = nullptr;    // [2*kMaxTiers] counter initial values
   int32_t* d_score = nullptr;
   int32_t* d_wf = nullptr;
   int32_t* d_status = nullptr;
@@ -143,7 +159,10 @@ struct bw_dbatch {
   int64_t runs_cap = 0;
   int32_t* d_gws = nullptr;
   int64_t gws_bytes = 0;
+  std::vector<int32_t> tier_count;   // initial pairs per tier (last schedule)
 };
+
+constexpr int kMaxTiers = 8;
 
 struct bw_aligner {
   bw_attr_t attr{};
@@ -152,14 +171,13 @@ struct bw_aligner {
   size_t lds_per_block_max = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  int32_t* d_counters = nullptr;   // [0] LDS queue, [1] spill count, [2] global queue
+  int32_t* d_counters = nullptr;   // per tier: {queue, count}
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
   // last single-pair result
   int status = 0, score = 0, wf_score = 0;
   std::vector<uint32_t> runs;
   std::vector<char> ops;
   std::vector<uint32_t> sam;
-  // per-aligner schedule
   int schedule = BW_SCHEDULE_LENGTH_SORTED;
 };
 
@@ -200,6 +218,10 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
   if (attr_in) attr = *attr_in; else bw_attr_default(&attr);
   Penalties pen;
   if (internal_penalties(attr.penalties, &pen) != 0) return nullptr;
+  if (attr.schedule < BW_SCHEDULE_INPUT_ORDER || attr.schedule > BW_SCHEDULE_LEARNED) {
+    set_error("unknown schedule");
+    return nullptr;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     set_error("no HIP device available");
@@ -226,7 +248,7 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
   a->lds_per_block_max = prop.sharedMemPerBlock;
   a->schedule = attr.schedule;
   if (hipSetDevice(a->dev) != hipSuccess || hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||
-      dalloc(&a->d_counters, 4) != hipSuccess) {
+      dalloc(&a->d_counters, 2 * kMaxTiers) != hipSuccess) {
     set_error("stream / counter allocation failed");
     delete a;
     return nullptr;
@@ -249,10 +271,91 @@ void bw_aligner_delete(bw_aligner_t* a) {
 void bw_dbatch_free(bw_dbatch_t* b) {
   if (!b) return;
   dfree(b->d_words); dfree(b->d_pwoff); dfree(b->d_plen); dfree(b->d_twoff); dfree(b->d_tlen);
-  dfree(b->d_order); dfree(b->d_spill); dfree(b->d_score); dfree(b->d_wf); dfree(b->d_status);
+  dfree(b->d_lists); dfree(b->d_init); dfree(b->d_score); dfree(b->d_wf); dfree(b->d_status);
   dfree(b->d_nruns); dfree(b->d_runs); dfree(b->d_runs_off); dfree(b->d_gws);
   delete b;
 }
+
+}  // extern "C"
+
+// Largest power of two dividing all of x, o, e (every reachable score is a
+// multiple of it), as a mask for the kernel's null-step shortcut.
+static int null_score_mask(const Penalties& q) {
+  int g = q.x | q.o | q.e;
+  g &= -g;                 // lowest set bit = largest power of two dividing all three
+  return g > 1 ? g - 1 : 0;
+}
+
+// Upper bound of a base-case score: 250 below a breakpoint, 500 on the
+// end-reached fallback, and the max(p,t) <= 100 whole-pair case
+// (min(p,t)*x + o + |p-t|*e, the bound WFA2 sizes its wavefront table with,
+// wavefront_components.c:81-100).
+static int base_score_bound(const Penalties& q) {
+  return std::max(bw::kRecoveryMinScore, 100 * q.x + q.o + 100 * q.e) + 1;
+}
+
+static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b) {
+  std::vector<Tier> tiers;
+  const Penalties& q = a->pen;
+  const int scope = q.scope;
+  const int smax = base_score_bound(q);
+  const int g = null_score_mask(q) + 1;
+  // tier 0: LDS workspace, ring width 32, sequences up to 512 bases staged
+  {
+    Tier t;
+    t.lds = true;
+    t.seq_cap = 512;
+    const int seq_words = t.seq_cap / 32 + 3;
+    t.L = make_layout(scope, 32, 0, bw::kWave, 128, 16, seq_words, false);
+    t.waves_per_block = bw::kWavesPerBlock;
+    const size_t block_bytes = (size_t)t.L.total * 4 * t.waves_per_block;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bw::biwfa_persistent<true>,
+                                                     bw::kWave * t.waves_per_block, block_bytes) != hipSuccess ||
+        nb <= 0)
+      nb = std::max<int>(1, (int)((160 * 1024) / block_bytes));
+    if (a->attr.waves_per_cu > 0) nb = std::max(1, std::min(nb, a->attr.waves_per_cu / t.waves_per_block));
+    t.blocks = a->cu_count * nb;
+    t.chunk = a->attr.queue_chunk > 0 ? a->attr.queue_chunk : 4;
+    tiers.push_back(t);
+  }
+  // global tiers: ring width 256, 1024, ... up to the batch maximum
+  const int wmax = next_pow2(std::max(b->max_pt + 8, 256));
+  const int64_t budget = (int64_t)24 << 30;
+  for (int w = 256;; w *= 4) {
+    const bool last = w >= wmax || (int)tiers.size() + 1 >= kMaxTiers;
+    const int wcap = last ? wmax : w;
+    Tier t;
+    t.lds = false;
+    const int width = std::min(2 * smax / q.e + 3, b->max_pt + 3);
+    const int64_t worst = (int64_t)3 * ((smax + g) / g) * width;
+    const int64_t arena = last ? worst : std::min<int64_t>(worst, (int64_t)3 * ((smax + g) / g) * std::min(wcap, width));
+    const int steps = std::min(b->max_pt, smax / std::max(1, std::min(q.x, q.e)) + 1);
+    t.L = make_layout(scope, wcap, (int)std::min<int64_t>(arena, (int64_t)1 << 30), smax + 2,
+                      2 * steps + 16, 64, 0, true);
+    t.stride = t.L.total;
+    const int64_t per_wave = t.stride * 4;
+    int64_t waves = std::max<int64_t>(bw::kWavesPerBlock, std::min<int64_t>(budget / per_wave, (int64_t)a->cu_count * 8));
+    t.blocks = (int)std::max<int64_t>(1, waves / bw::kWavesPerBlock);
+    t.chunk = 1;
+    tiers.push_back(t);
+    if (last || (int)tiers.size() >= kMaxTiers) break;
+  }
+  return tiers;
+}
+
+// Initial tier of a pair: tier 0 when it can be staged in LDS, else the first
+// global tier whose ring is at least `width_guess` wide.
+static int initial_tier(const bw_dbatch_t* b, int pair, double width_guess) {
+  const int p = b->h_plen[pair], t = b->h_tlen[pair];
+  const auto& T = b->tiers;
+  if (std::max(p, t) <= T[0].seq_cap && width_guess <= T[0].L.wcap) return 0;
+  for (size_t i = 1; i < T.size(); ++i)
+    if (T[i].L.wcap >= width_guess || i + 1 == T.size()) return (int)i;
+  return (int)T.size() - 1;
+}
+
+extern "C" {
 
 static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out) {
   *out = nullptr;
@@ -261,7 +364,7 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
     set_error("invalid batch");
     return BW_STATUS_INVALID_ARG;
   }
-  if (in->num_pairs > INT32_MAX / 2) {
+  if (in->num_pairs > INT32_MAX / kMaxTiers) {
     set_error("too many pairs in one batch");
     return BW_STATUS_INVALID_ARG;
   }
@@ -276,8 +379,8 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
   for (int64_t i = 0; i < n; ++i) {
     const int p = b->h_plen[i], t = b->h_tlen[i];
     const int64_t pw = in->pattern_word_offset[i], tw = in->text_word_offset[i];
-    if (p < 0 || t < 0 || pw < 1 || tw < 1 || pw + (p + 31) / 32 + 2 > in->num_words ||
-        tw + (t + 31) / 32 + 2 > in->num_words) {
+    if (p < 0 || t < 0 || (int64_t)p + t > (1 << 28) || pw < 1 || tw < 1 ||
+        pw + (p + 31) / 32 + 2 > in->num_words || tw + (t + 31) / 32 + 2 > in->num_words) {
       set_error("pair " + std::to_string(i) + ": bad length or word offset (need one pad word "
                 "before and two after every sequence)");
       bw_dbatch_free(b);
@@ -289,25 +392,32 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
     runs_off[i + 1] = runs_off[i] + (int64_t)p + t + 1;
   }
   b->runs_cap = runs_off[n];
+  b->tiers = plan_tiers(a, b);
   auto fail = [&](hipError_t e) {
     set_error(std::string("device allocation/copy failed: ") + hipGetErrorString(e));
     bw_dbatch_free(b);
     return BW_STATUS_HIP_ERROR;
   };
   hipError_t e;
+  const size_t nt = b->tiers.size();
   if ((e = dalloc(&b->d_words, in->num_words)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_pwoff, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_plen, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_twoff, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_tlen, n)) != hipSuccess) return fail(e);
-  if ((e = dalloc(&b->d_order, n)) != hipSuccess) return fail(e);
-  if ((e = dalloc(&b->d_spill, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_lists, nt * (size_t)std::max<int64_t>(n, 1))) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_init, 2 * kMaxTiers)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_score, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_wf, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_status, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_nruns, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_runs, b->runs_cap)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_runs_off, n + 1)) != hipSuccess) return fail(e);
+  int64_t gbytes = 0;
+  for (const Tier& t : b->tiers)
+    if (!t.lds) gbytes = std::max<int64_t>(gbytes, (int64_t)t.blocks * bw::kWavesPerBlock * t.stride * 4);
+  if ((e = hipMalloc(reinterpret_cast<void**>(&b->d_gws), std::max<int64_t>(gbytes, 64))) != hipSuccess) return fail(e);
+  b->gws_bytes = gbytes;
   hipStream_t s = a->stream;
   if ((e = hipMemcpyAsync(b->d_words, in->words, sizeof(uint64_t) * in->num_words, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
   if (n > 0) {
@@ -319,7 +429,8 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
   if ((e = hipMemcpyAsync(b->d_runs_off, runs_off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e);
   *out = b;
-  const int st = bw_set_schedule(a, b, a->schedule, nullptr);
+  const int sched = a->schedule == BW_SCHEDULE_LEARNED ? BW_SCHEDULE_LENGTH_SORTED : a->schedule;
+  const int st = bw_set_schedule(a, b, sched, nullptr);
   if (st != 0) {
     bw_dbatch_free(b);
     *out = nullptr;
@@ -334,86 +445,59 @@ bw_dbatch_t* bw_dbatch_upload(bw_aligner_t* a, const bw_batch_t* in) {
   return b;
 }
 
-int bw_set_schedule(bw_aligner_t* a, bw_dbatch_t* b, int schedule, const float* cost) {
+int bw_set_schedule(bw_aligner_t* a, bw_dbatch_t* b, int schedule, const float* predicted_score) {
   if (!a || !b) return BW_STATUS_INVALID_ARG;
-  if (schedule == BW_SCHEDULE_LEARNED && !cost) {
-    set_error("learned schedule needs one predicted cost per pair");
+  if (schedule == BW_SCHEDULE_LEARNED && !predicted_score) {
+    set_error("the learned schedule needs one predicted score per pair");
     return BW_STATUS_INVALID_ARG;
   }
-  const int64_t n = b->n;
-  std::vector<int32_t> order(n);
-  std::iota(order.begin(), order.end(), 0);
-  if (schedule == BW_SCHEDULE_LENGTH_SORTED) {
-    std::stable_sort(order.begin(), order.end(), [&](int32_t i, int32_t j) {
-      return (b->h_plen[i] + b->h_tlen[i]) > (b->h_plen[j] + b->h_tlen[j]);
-    });
-  } else if (schedule == BW_SCHEDULE_LEARNED) {
-    std::stable_sort(order.begin(), order.end(), [&](int32_t i, int32_t j) { return cost[i] > cost[j]; });
-  } else if (schedule != BW_SCHEDULE_INPUT_ORDER) {
+  if (schedule < BW_SCHEDULE_INPUT_ORDER || schedule > BW_SCHEDULE_LEARNED) {
     set_error("unknown schedule");
     return BW_STATUS_INVALID_ARG;
   }
-  a->schedule = schedule;
-  HIP_TRY(hipSetDevice(a->dev));
-  if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(b->d_order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, a->stream));
-    HIP_TRY(hipStreamSynchronize(a->stream));
-  }
-  return 0;
-}
-
-// Launch plan for one batch.
-struct Plan {
-  bool use_lds = false;
-  bw::WsLayout L_lds{}, L_glb{};
-  int lds_waves_per_block = 4, lds_blocks = 0;
-  int glb_blocks = 0;
-  int64_t glb_stride = 0;
-  int chunk = 4;
-};
-
-static Plan make_plan(const bw_aligner_t* a, const bw_dbatch_t* b) {
-  Plan pl;
+  const int64_t n = b->n;
   const Penalties& q = a->pen;
-  const int pt = b->max_pt;
-  const int scope = q.scope;
-  // --- LDS instantiation: short pairs (ring width 32/64 diagonals) ---
-  const int maxlen = std::max(b->max_p, b->max_t);
-  if (maxlen <= 2048) {
-    const int wcap = (maxlen <= 400) ? 32 : 64;
-    const int seq_words = (maxlen + 31) / 32 + 3;
-    pl.L_lds = make_layout(scope, wcap, 0, 64, 128, 16, seq_words, false);
-    const size_t bytes_per_wave = (size_t)pl.L_lds.total * 4;
-    int wpb = 4;
-    while (wpb > 1 && bytes_per_wave * wpb > 64 * 1024) wpb >>= 1;
-    if (bytes_per_wave * wpb <= a->lds_per_block_max) {
-      pl.use_lds = true;
-      pl.lds_waves_per_block = wpb;
-      int blocks_per_cu = (int)((160 * 1024) / (bytes_per_wave * wpb));
-      blocks_per_cu = std::max(1, std::min(blocks_per_cu, 32 / wpb));
-      if (a->attr.waves_per_cu > 0) blocks_per_cu = std::max(1, a->attr.waves_per_cu / wpb);
-      pl.lds_blocks = a->cu_count * blocks_per_cu;
+  // cost key (larger first) and initial tier of every pair
+  std::vector<double> key(n);
+  std::vector<int> tier(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const double pt = (double)b->h_plen[i] + b->h_tlen[i];
+    if (schedule == BW_SCHEDULE_LEARNED) {
+      // BiWFA work grows with the square of the score (both directions of every
+      // recursion level cover ~s/e diagonals for ~s/2 scores); the widest ring a
+      // pair needs is about its per-direction score over e.
+      const double s = std::max(0.0, (double)predicted_score[i]) * ((q.um < 0) ? 2.0 : 1.0);
+      key[i] = s * s / q.e + 4.0 * pt;
+      tier[i] = initial_tier(b, (int)i, std::min(pt + 8.0, 1.25 * s / q.e + 8.0));
+    } else {
+      // no score estimate: pairs that fit the LDS staging start in tier 0, longer
+      // ones in the first global tier as wide as a quarter of their length
+      key[i] = (schedule == BW_SCHEDULE_LENGTH_SORTED) ? pt : 0.0;
+      const bool stage = std::max(b->h_plen[i], b->h_tlen[i]) <= b->tiers[0].seq_cap;
+      tier[i] = initial_tier(b, (int)i, stage ? 0.0 : std::min(pt + 8.0, 0.25 * pt + 64.0));
     }
   }
-  // --- global instantiation: capacities from the batch maxima ---
-  const int wcap = next_pow2(pt + 8);
-  const int min_length_bound = 100 * q.x + q.o + 100 * q.e;  // max(p,t) <= 100
-  int smax = std::max(bw::kRecoveryMinScore, bw::kFallbackMinScore) + 1;
-  if (std::max(b->max_p, b->max_t) <= bw::kFallbackMinLength) smax = std::max(smax, min_length_bound + 1);
-  else smax = std::max(smax, min_length_bound + 1);
-  const int width = std::min(2 * smax / q.e + 3, pt + 3);
-  const int64_t arena = std::min<int64_t>((int64_t)3 * (smax + 1) * width, (int64_t)1 << 28);
-  const int steps = std::min(pt, smax / std::max(1, std::min(q.x, q.e)) + 1);
-  const int rcap = 2 * steps + 16;
-  pl.L_glb = make_layout(scope, wcap, (int)arena, smax + 2, rcap, 64, 0, true);
-  pl.glb_stride = pl.L_glb.total;
-  const int64_t bytes_per_wave = pl.glb_stride * 4;
-  const int64_t budget = (int64_t)16 << 30;
-  int64_t waves = std::max<int64_t>(1, std::min<int64_t>(budget / bytes_per_wave, (int64_t)a->cu_count * 8));
-  if (!pl.use_lds) waves = std::min<int64_t>(waves, std::max<int64_t>(1, b->n));
-  pl.glb_blocks = (int)std::max<int64_t>(1, waves / bw::kWavesPerBlock);
-  pl.chunk = a->attr.queue_chunk > 0 ? a->attr.queue_chunk : (pl.use_lds ? 4 : 1);
-  return pl;
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  if (schedule != BW_SCHEDULE_INPUT_ORDER)
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return key[x] > key[y]; });
+  const size_t nt = b->tiers.size();
+  std::vector<std::vector<int32_t>> lists(nt);
+  for (int32_t i : order) lists[tier[i]].push_back(i);
+  a->schedule = schedule;
+  HIP_TRY(hipSetDevice(a->dev));
+  std::vector<int32_t> init(2 * kMaxTiers, 0);
+  b->tier_count.assign(nt, 0);
+  for (size_t t = 0; t < nt; ++t) {
+    init[2 * t + 1] = (int32_t)lists[t].size();
+    b->tier_count[t] = (int32_t)lists[t].size();
+    if (!lists[t].empty())
+      HIP_TRY(hipMemcpyAsync(b->d_lists + t * (size_t)n, lists[t].data(), 4 * lists[t].size(),
+                             hipMemcpyHostToDevice, a->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_init, init.data(), 4 * init.size(), hipMemcpyHostToDevice, a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  return 0;
 }
 
 static bw::KParams base_params(const bw_aligner_t* a, const bw_dbatch_t* b) {
@@ -423,6 +507,7 @@ static bw::KParams base_params(const bw_aligner_t* a, const bw_dbatch_t* b) {
   p.t_woff = b->d_twoff; p.t_len = b->d_tlen;
   p.x = a->pen.x; p.o = a->pen.o; p.e = a->pen.e; p.scope = a->pen.scope;
   p.um = a->pen.um; p.ux = a->pen.ux; p.uo = a->pen.uo; p.ue = a->pen.ue;
+  p.gmask = null_score_mask(a->pen);
   p.score_only = a->attr.scope == BW_SCOPE_SCORE;
   p.max_steps = a->attr.max_alignment_steps > 0 ? a->attr.max_alignment_steps : INT32_MAX;
   p.score = b->d_score; p.wf_score = b->d_wf; p.status = b->d_status;
@@ -434,57 +519,35 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
   if (!a || !b) return BW_STATUS_INVALID_ARG;
   HIP_TRY(hipSetDevice(a->dev));
   if (b->n == 0) return 0;
-  const Plan pl = make_plan(a, b);
-  // global workspace (kept with the batch, grown on demand)
-  const int64_t gbytes = (int64_t)pl.glb_blocks * bw::kWavesPerBlock * pl.glb_stride * 4;
-  if (gbytes > b->gws_bytes) {
-    dfree(b->d_gws);
-    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b->d_gws), gbytes));
-    b->gws_bytes = gbytes;
-  }
   hipStream_t s = a->stream;
+  const int nt = (int)b->tiers.size();
   HIP_TRY(hipEventRecord(a->ev[0], s));
-  hipLaunchKernelGGL(bw::biwfa_reset, dim3(1), dim3(64), 0, s, a->d_counters, 4);
-  bw::KParams p = base_params(a, b);
-  p.gws = b->d_gws;
-  p.gws_stride = pl.glb_stride;
-  if (pl.use_lds) {
-    p.order = b->d_order;
-    p.work_list = nullptr;
-    p.work_count = nullptr;
-    p.num_work = (int32_t)b->n;
-    p.queue = a->d_counters + 0;
-    p.chunk = pl.chunk;
-    p.spill_count = a->d_counters + 1;
-    p.spill_list = b->d_spill;
-    p.L = pl.L_lds;
-    const size_t lds = (size_t)pl.L_lds.total * 4 * pl.lds_waves_per_block;
-    HIP_TRY(hipEventRecord(a->ev[1], s));
-    hipLaunchKernelGGL(bw::biwfa_persistent<true>, dim3(pl.lds_blocks),
-                       dim3(bw::kWave * pl.lds_waves_per_block), lds, s, p);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(a->ev[2], s));
-    // spilled pairs
-    p.order = nullptr;
-    p.work_list = b->d_spill;
-    p.work_count = a->d_counters + 1;
-    p.num_work = 0;
-  } else {
-    p.order = b->d_order;
-    p.work_list = nullptr;
-    p.work_count = nullptr;
-    p.num_work = (int32_t)b->n;
-    HIP_TRY(hipEventRecord(a->ev[1], s));
-  }
-  p.queue = a->d_counters + 2;
-  p.chunk = 1;
-  p.spill_count = nullptr;
-  p.spill_list = nullptr;
-  p.L = pl.L_glb;
-  hipLaunchKernelGGL(bw::biwfa_persistent<false>, dim3(pl.glb_blocks), dim3(bw::kWave * bw::kWavesPerBlock),
-                     0, s, p);
+  hipLaunchKernelGGL(bw::biwfa_reset, dim3(1), dim3(64), 0, s, a->d_counters, b->d_init, 2 * kMaxTiers);
   HIP_TRY(hipGetLastError());
-  if (!pl.use_lds) HIP_TRY(hipEventRecord(a->ev[2], s));
+  HIP_TRY(hipEventRecord(a->ev[1], s));
+  for (int t = 0; t < nt; ++t) {
+    const Tier& T = b->tiers[t];
+    bw::KParams p = base_params(a, b);
+    p.list = b->d_lists + (size_t)t * b->n;
+    p.count = a->d_counters + 2 * t + 1;
+    p.queue = a->d_counters + 2 * t;
+    p.chunk = T.chunk;
+    p.next_list = (t + 1 < nt) ? b->d_lists + (size_t)(t + 1) * b->n : nullptr;
+    p.next_count = (t + 1 < nt) ? a->d_counters + 2 * (t + 1) + 1 : nullptr;
+    p.L = T.L;
+    p.gws = b->d_gws;
+    p.gws_stride = T.stride;
+    if (T.lds) {
+      const size_t lds = (size_t)T.L.total * 4 * T.waves_per_block;
+      hipLaunchKernelGGL(bw::biwfa_persistent<true>, dim3(T.blocks), dim3(bw::kWave * T.waves_per_block),
+                         lds, s, p);
+    } else {
+      hipLaunchKernelGGL(bw::biwfa_persistent<false>, dim3(T.blocks), dim3(bw::kWave * bw::kWavesPerBlock),
+                         0, s, p);
+    }
+    HIP_TRY(hipGetLastError());
+    if (t == 0) HIP_TRY(hipEventRecord(a->ev[2], s));
+  }
   HIP_TRY(hipEventRecord(a->ev[3], s));
   return 0;
 }

@@ -199,6 +199,32 @@ int bw_last_timing(const bw_aligner_t* aligner, float* total_ms, float* main_ms,
 int bw_set_schedule(bw_aligner_t* aligner, bw_dbatch_t* dbatch, int schedule,
                     const float* predicted_cost);
 
+/* ---------------- (v) learned cost model ---------------- */
+
+/* No counterpart in the reference: a 4 -> 64 -> 64 -> 1 ReLU MLP (BF16 weights,
+ * FP32 accumulation) that predicts a pair's alignment score from
+ *   log2(p+1)/16, log2(t+1)/16, log2(|p-t|+1)/16, fraction of 16 sampled 12-mers
+ * of the pattern found near their expected text position;
+ * target log2(1 + score)/16.  Trained and evaluated by the library's own HIP
+ * kernels; its predictions feed bw_set_schedule(BW_SCHEDULE_LEARNED). */
+typedef struct bw_mlp bw_mlp_t;
+int bw_mlp_num_params(void);
+bw_mlp_t* bw_mlp_new(int device, uint64_t seed);
+void bw_mlp_free(bw_mlp_t* mlp);
+const char* bw_mlp_last_error(void);
+/* FP32 master parameters and their BF16-rounded copy (either may be NULL). */
+int bw_mlp_get_params(bw_mlp_t* mlp, float* master, float* bf16_weights);
+int bw_mlp_set_params(bw_mlp_t* mlp, const float* master);
+/* X: n x 4 features (host), y: n outputs (host). */
+int bw_mlp_predict(bw_mlp_t* mlp, const float* X, int64_t n, float* y);
+/* Gradient of the mean squared error w.r.t. the BF16 weights (host output). */
+int bw_mlp_gradient(bw_mlp_t* mlp, const float* X, const float* y, int64_t n, float* grad);
+/* Mini-batch Adam on the device; *final_loss = full-set MSE afterwards. */
+int bw_mlp_train(bw_mlp_t* mlp, const float* X, const float* y, int64_t n, int epochs, int batch,
+                 float lr, uint64_t seed, float* final_loss);
+/* The four features of every pair of a packed batch, computed on the device. */
+int bw_batch_features(int device, const bw_batch_t* batch, float* X);
+
 /* ---------------- generator / packing ---------------- */
 
 typedef struct {

@@ -177,7 +177,7 @@ def test_schedules_and_determinism(aligners, oracle):
     db = a.upload(ds.pack())
     outs = []
     for sched, cost in ((bw.SCHEDULE_INPUT_ORDER, None), (bw.SCHEDULE_LENGTH_SORTED, None),
-                        (bw.SCHEDULE_LEARNED, np.random.default_rng(0).random(200))):
+                        (bw.SCHEDULE_LEARNED, 400.0 * np.random.default_rng(0).random(200))):
         db.set_schedule(sched, cost)
         db.align()
         outs.append(db.download())
@@ -185,6 +185,26 @@ def test_schedules_and_determinism(aligners, oracle):
         assert np.array_equal(r.score, outs[0].score)
         assert np.array_equal(r.cigar_offset, outs[0].cigar_offset)
         assert np.array_equal(r.cigar_runs, outs[0].cigar_runs)
+    db.free()
+
+
+def test_spill_chain_under_wrong_predictions(aligners, oracle):
+    """Long divergent pairs predicted at score 0 start in the smallest workspace tier
+    and must climb the spill chain tier by tier; results stay bit-exact."""
+    import dataclasses
+    _say("spill chain")
+    c3 = bw.generate_dataset(bw.CONFIGS["C3"], num_pairs=6).pairs()
+    c5 = bw.generate_dataset(dataclasses.replace(bw.CONFIGS["C5"], len_min=12_000, len_max=12_000,
+                                                 num_edits=1_800), num_pairs=3).pairs()
+    c1 = bw.generate_dataset(bw.CONFIGS["C1"], num_pairs=40).pairs()
+    pairs = c3 + c5 + c1 + EDGE_PAIRS
+    ref = oracle.align(pairs, (0, 4, 6, 2), threads=16)
+    a = aligners((0, 4, 6, 2))
+    db = a.upload(bw.pack_pairs(pairs))
+    db.set_schedule(bw.SCHEDULE_LEARNED, np.zeros(len(pairs)))
+    db.align()
+    res = db.download()
+    _compare(res, ref.alg_score, [ref.cigar(i) for i in range(len(pairs))], "spill-chain")
     db.free()
 
 
