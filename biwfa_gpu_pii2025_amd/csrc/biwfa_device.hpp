@@ -23,23 +23,39 @@ constexpr int kStGuard = -405;       // termination guard tripped (never in a co
 
 constexpr int kMaxScope = 32;          // max(o+e, x)+1 supported by the ring layout
 constexpr int kWave = 64;              // gfx950 wavefront size
-constexpr int kWavesPerBlock = 4;      // independent waves per 256-thread workgroup
+constexpr int kWavesPerBlock = 4;      // independent single-wave aligners per workgroup
+constexpr int kMaxTeam = 16;           // waves cooperating on one pair (one workgroup)
+constexpr int kRedSlots = 3 * kMaxScope;   // team reduction values per wave
 
-// Per-wave workspace layout, in int32 units from the wave's workspace base.
+// Kernel configuration: kLds keeps the whole workspace and both sequences in
+// LDS (one wave per pair); otherwise the workspace is a global slab shared by a
+// team of kNW waves (kNW > 1: the team is one workgroup).
+template <bool L, int NW>
+struct Cfg {
+  static constexpr bool kLds = L;
+  static constexpr int kNW = NW;
+};
+
+// Workspace layout of one aligner (a wave, or a team of waves), in int32 units
+// from its workspace base.  The ring / arena is shared by the team; every wave
+// has a private region (priv + wave * priv_ints) holding its copy of the
+// base-case headers, the backtrace buffer and the recursion stack, so that the
+// wave-uniform bookkeeping never needs cross-wave synchronisation.
 struct WsLayout {
-  int ring_hdr;     // 2 dirs x scope x 3 comps x {lo, hi, present, pad}
   int data;         // ring data (2 x scope x 3 x wcap)  UNION  base-case arena
   int wcap;         // ring width capacity (power of two)
   int arena_cap;    // ints available at `data` for the base-case arena
-  int bhdr;         // base-case headers: hcap scores x 3 comps x {lo, hi, kbase}
+  int priv;         // first private region
+  int priv_ints;    // ints per private region
+  int bhdr;         // (private) base-case headers: hcap scores x 3 comps x {lo, hi, kbase}
   int hcap;         // max base-case score + 1
-  int runs_tmp;     // reversed runs of one backtrace (uint32)
+  int runs_tmp;     // (private) reversed runs of one backtrace (uint32)
   int rcap;
-  int stack;        // recursion stack: scap entries x 8 ints
+  int stack;        // (private) recursion stack: scap entries x 8 ints
   int scap;
   int seq;          // staged 2-bit sequences (LDS path): P then T, each with pad words
   int seq_words;    // words available for each of P and T (including 3 pad words)
-  int total;        // ints per wave
+  int total;        // ints per aligner
 };
 
 // Kernel parameters (passed by value).
@@ -74,8 +90,9 @@ struct KParams {
   const int64_t* runs_off;       // [N+1] run capacity prefix
   int32_t* nruns;
   // workspace
-  int32_t* gws;                  // global workspace (global kernel)
-  int64_t gws_stride;            // ints per wave
+  int32_t* gws;                  // global workspace (global kernels)
+  int64_t gws_stride;            // ints per aligner (wave or team)
+  int32_t seq_lds_words;         // team kernels: LDS words per sequence for staging (0: none)
   WsLayout L;
 };
 

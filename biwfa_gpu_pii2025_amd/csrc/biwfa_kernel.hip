@@ -1,4 +1,5 @@
-// biwfa_kernel.hip -- gfx950 bidirectional WFA (score + CIGAR), one wave64 per pair.
+// biwfa_kernel.hip -- gfx950 bidirectional WFA (score + CIGAR): one wave64, or a
+// team of wave64s in one workgroup, per pair.
 //
 // Behaviour follows WFA2-lib v2.3 `wavefront_align` in exact BiWFA mode
 // (memory_mode = ultralow, heuristic = none, gap-affine):
@@ -13,9 +14,8 @@
 //                         wavefront/wavefront_backtrace.c:38-529
 //
 // Device mapping:
-//   * A persistent grid of workgroups; each wave is an independent aligner that
-//     pulls pairs from a device work queue (atomic counter over a schedule
-//     order).  No cross-wave synchronisation.
+//   * A persistent grid of workgroups; each aligner (a wave, or a team) pulls
+//     pairs from a device work queue (atomic counter over a schedule order).
 //   * Diagonals are spread over the 64 lanes; scores, limits and the recursion
 //     state are wave-uniform (scalar registers).
 //   * Wavefront limits (lo/hi per component) are kept in vector registers, one
@@ -27,9 +27,13 @@
 //     the ring storage.
 //   * Sequences are 2-bit packed; extension compares 32 bases per 64-bit word
 //     (xor + count-trailing-zeros forward, count-leading-zeros backward).
-//   * Two instantiations: kLds keeps the workspace and both sequences in LDS; a
-//     pair that outgrows an LDS capacity is pushed to a spill list and re-run by
-//     the global-workspace instantiation with large capacities.
+//   * Instantiations: Cfg<true,1> keeps the workspace and both sequences in LDS,
+//     one wave per pair; a pair that outgrows an LDS capacity is pushed to a
+//     spill list and re-run by a global-workspace instantiation with larger
+//     capacities.  Cfg<false,NW> with NW > 1 aligns one pair per workgroup of
+//     NW waves (a team): each step's diagonals are interleaved over the waves in
+//     64-diagonal chunks, and the per-step reductions (trimmed limits, maximum
+//     antidiagonal, first overlap hit) go through LDS with one barrier each.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
@@ -68,6 +72,15 @@ DI void wave_sync() {
   }
 }
 
+// Orders the aligner's workspace writes before later reads by any of its
+// lanes: a wave barrier for a single-wave aligner, a workgroup barrier (with
+// its release/acquire fences) for a team.
+template <class K>
+DI void team_sync() {
+  if constexpr (K::kNW == 1) wave_sync<K::kLds>();
+  else __syncthreads();
+}
+
 // Max over the wave of non-negative values (DPP row scans, then row broadcasts).
 DI int wave_max_nonneg(int v) {
   v = imax(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));  // row_shr:1
@@ -104,12 +117,17 @@ struct BP {                 // wavefront_bialigner.h:43-54
   int score, sf, sr, kf, kr, of, orr, comp;
 };
 
-template <bool kLds>
+template <class K>
 struct Ctx {
-  int32_t* ws;
+  int32_t* ws;    // shared workspace (ring / arena)
+  int32_t* pv;    // this wave's private region (base headers, backtrace runs, stack)
   const uint64_t* P;
   const uint64_t* T;
   int lane;
+  int wid;        // wave index within the team (0 for a single-wave aligner)
+  bool leader;    // lane 0 of wave 0: the one lane that writes results
+  int32_t* tred;  // team reduction scratch in LDS (team kernels)
+  int tph;        // its double-buffer phase
   WsLayout L;
   int x, o, e, scope, max_steps;
   int um, ux, uo, ue;
@@ -130,8 +148,8 @@ struct Ctx {
 // Upper bound of the optimal score of a plen x tlen sub-problem plus the
 // slack the breakpoint search may run past it; a search that goes beyond it
 // can only be a defect, and is stopped with kStGuard instead of looping.
-template <bool kLds>
-DI int score_guard(const Ctx<kLds>& c, int plen, int tlen) {
+template <class K>
+DI int score_guard(const Ctx<K>& c, int plen, int tlen) {
   const int mn = imin(plen, tlen), df = imax(plen, tlen) - mn;
   return mn * c.x + c.o + df * c.e + c.o + 2 * c.scope + 16;
 }
@@ -154,17 +172,80 @@ DI int slot_back(int slot, int delta, int scope) {
 }
 
 // ---------------------------------------------------------------------------
+// team reductions (kNW > 1).  Every wave of a team runs the same uniform
+// control flow on identical copies of the aligner state; values that depend on
+// the diagonals a wave owns are combined through a double-buffered LDS area
+// (kNW x kRedSlots ints per phase).  Two phases suffice: a wave can only write
+// a phase again after passing the barrier of the other phase, which every wave
+// reaches only once its reads of the first are done.
+
+template <class K>
+DI int32_t* team_buf(Ctx<K>& c) {
+  int32_t* b = c.tred + c.tph * (K::kNW * kRedSlots);
+  c.tph ^= 1;
+  return b;
+}
+
+// In-place max over the team of up to six wave-uniform values (one barrier).
+// Lane j of every wave publishes value j; after the barrier lane j folds the
+// waves' copies of value j.  The lane's value is picked with masking
+// arithmetic: a select chain would become a select of the arguments' addresses
+// and move them to scratch.
+template <class K>
+DI void team_max6(Ctx<K>& c, int& v0, int& v1, int& v2, int& v3, int& v4, int& v5) {
+  int32_t* b = team_buf(c);
+  const int l = c.lane;
+  const int a0 = v0, a1 = v1, a2 = v2, a3 = v3, a4 = v4, a5 = v5;
+  const int mine = (a0 & -(int)(l == 0)) | (a1 & -(int)(l == 1)) | (a2 & -(int)(l == 2)) |
+                   (a3 & -(int)(l == 3)) | (a4 & -(int)(l == 4)) | (a5 & -(int)(l == 5));
+  if (l < 6) b[c.wid * kRedSlots + l] = mine;
+  __syncthreads();
+  int r = INT_MIN;
+  if (l < 6) {
+#pragma unroll
+    for (int w = 0; w < K::kNW; ++w) r = imax(r, b[w * kRedSlots + l]);
+  }
+  v0 = rdl(r, 0); v1 = rdl(r, 1); v2 = rdl(r, 2);
+  v3 = rdl(r, 3); v4 = rdl(r, 4); v5 = rdl(r, 5);
+}
+template <class K>
+DI int team_max1(Ctx<K>& c, int v) {
+  int32_t* b = team_buf(c);
+  if (c.lane == 0) b[c.wid * kRedSlots] = v;
+  __syncthreads();
+  int r = INT_MIN;
+  if (c.lane < K::kNW) r = b[c.lane * kRedSlots];
+  r = imax(r, __builtin_amdgcn_update_dpp(INT_MIN, r, 0x111, 0xf, 0xf, false));  // row_shr:1
+  r = imax(r, __builtin_amdgcn_update_dpp(INT_MIN, r, 0x112, 0xf, 0xf, false));  // row_shr:2
+  r = imax(r, __builtin_amdgcn_update_dpp(INT_MIN, r, 0x114, 0xf, 0xf, false));  // row_shr:4
+  r = imax(r, __builtin_amdgcn_update_dpp(INT_MIN, r, 0x118, 0xf, 0xf, false));  // row_shr:8
+  return rdl(r, 15);
+}
+
+// Min over the team of the values the waves stored at slot j of `buf`
+// (lanes 0..15 load one wave's value each, then a row-wide DPP scan).
+template <class K>
+DI int team_min_slot(const Ctx<K>& c, const int32_t* buf, int j) {
+  int v = (c.lane < K::kNW) ? buf[c.lane * kRedSlots + j] : INT_MAX;
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  return rdl(v, 15);
+}
+
+// ---------------------------------------------------------------------------
 // headers
 
-template <bool kLds>
-DI int ring_base(const Ctx<kLds>& c, int ring, int slot, int comp) {
+template <class K>
+DI int ring_base(const Ctx<K>& c, int ring, int slot, int comp) {
   return c.L.data + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
 }
 
 // Presence bit of (ring slot | base score) for component `comp`: ring slots use
 // bit ring*32 + slot, base-case scores (LDS kernel) bit score.
-template <bool kLds>
-DI bool is_present(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp) {
+template <class K>
+DI bool is_present(const Ctx<K>& c, const Dir& d, int score, int slot, int comp) {
   if (d.ring >= 0) return (sel3u(c.rpres, comp) >> (d.ring * 32 + slot)) & 1ull;
   return (sel3u(c.bpres, comp) >> score) & 1ull;
 }
@@ -172,8 +253,8 @@ DI bool is_present(const Ctx<kLds>& c, const Dir& d, int score, int slot, int co
 // Component `comp` at `score` (ring slot `slot`), WFA2's
 // wavefront_compute_get_*wavefront (wavefront_compute.c:266-305).  *present
 // reports a non-NULL WFA2 pointer (possibly with lo > hi).
-template <bool kLds>
-DI Wf get_wf(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, int* present) {
+template <class K>
+DI Wf get_wf(const Ctx<K>& c, const Dir& d, int score, int slot, int comp, int* present) {
   *present = 0;
   if (score < 0) return wf_null();
   if (d.ring >= 0) {
@@ -186,24 +267,24 @@ DI Wf get_wf(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, in
     return Wf{lo, hi, ring_base(c, d.ring, slot, comp), c.L.wcap - 1};
   }
   int lo, hi, kb;
-  if (kLds) {
+  if (K::kLds) {
     if (!is_present(c, d, score, slot, comp)) return wf_null();
     lo = rdl(sel3(c.blo, comp), score);
     hi = rdl(sel3(c.bhi, comp), score);
     kb = rdl(sel3(c.bkb, comp), score);
   } else {
     const int h = c.L.bhdr + (score * 3 + comp) * 3;
-    lo = uni(c.ws[h]);
-    hi = uni(c.ws[h + 1]);
-    kb = uni(c.ws[h + 2]);
+    lo = uni(c.pv[h]);
+    hi = uni(c.pv[h + 1]);
+    kb = uni(c.pv[h + 2]);
   }
   if (lo == kAbsent) return wf_null();
   *present = 1;
   return Wf{lo, hi, kb, -1};
 }
 
-template <bool kLds>
-DI Wf get_in(const Ctx<kLds>& c, const Dir& d, int score, int slot, int comp) {
+template <class K>
+DI Wf get_in(const Ctx<K>& c, const Dir& d, int score, int slot, int comp) {
   int present;
   Wf w = get_wf(c, d, score, slot, comp, &present);
   if (w.lo > w.hi) return wf_null();
@@ -226,8 +307,8 @@ DI void bit_set(uint64_t (&m)[3], int comp, int bit, bool on) {
 }
 
 // lo == kAbsent marks an unallocated component
-template <bool kLds>
-DI void set_hdr(Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, int lo, int hi, int kb) {
+template <class K>
+DI void set_hdr(Ctx<K>& c, const Dir& d, int score, int slot, int comp, int lo, int hi, int kb) {
   const bool on = lo != kAbsent;
   if (d.ring >= 0) {
     bit_set(c.rpres, comp, d.ring * 32 + slot, on);
@@ -236,7 +317,7 @@ DI void set_hdr(Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, int l
       lane_set(c.rlo, comp, c.lane, j, lo);
       lane_set(c.rhi, comp, c.lane, j, hi);
     }
-  } else if (kLds) {
+  } else if (K::kLds) {
     bit_set(c.bpres, comp, score, on);
     if (on) {
       lane_set(c.blo, comp, c.lane, score, lo);
@@ -245,9 +326,9 @@ DI void set_hdr(Ctx<kLds>& c, const Dir& d, int score, int slot, int comp, int l
     }
   } else {
     const int h = c.L.bhdr + (score * 3 + comp) * 3;
-    c.ws[h] = lo;
-    c.ws[h + 1] = hi;
-    c.ws[h + 2] = kb;
+    c.pv[h] = lo;
+    c.pv[h + 1] = hi;
+    c.pv[h + 2] = kb;
   }
 }
 
@@ -263,8 +344,8 @@ DI uint64_t window(const uint64_t* s, int pos) {       // bases [pos, pos+32)
 
 // Longest common extension from (v,h) of the aligner's sub-problem, 32 bases
 // per 64-bit compare, clamped to the sub-problem's end.
-template <bool kLds>
-DI int lcp(const Ctx<kLds>& c, const Dir& d, int v, int h) {
+template <class K>
+DI int lcp(const Ctx<K>& c, const Dir& d, int v, int h) {
   const int lim = imin(d.plen - v, d.tlen - h);
   int n = 0;
   if (!d.rev) {
@@ -290,16 +371,16 @@ DI int lcp(const Ctx<kLds>& c, const Dir& d, int v, int h) {
 // compute: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
 // `slot` is score % scope for ring aligners.
 
-template <bool kLds>
-DI void null_step(Ctx<kLds>& c, Dir& d, int s, int slot) {
+template <class K>
+DI void null_step(Ctx<K>& c, Dir& d, int s, int slot) {
   d.null_steps++;
   set_hdr(c, d, s, slot, CM, kAbsent, 0, 0);
   set_hdr(c, d, s, slot, CI, kAbsent, 0, 0);
   set_hdr(c, d, s, slot, CD, kAbsent, 0, 0);
 }
 
-template <bool kLds>
-DI void compute_step(Ctx<kLds>& c, Dir& d, int s, int slot) {
+template <class K>
+DI void compute_step(Ctx<K>& c, Dir& d, int s, int slot) {
   const int sc = c.scope;
   if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return; }
   // With penalties that share a power-of-two factor g, only multiples of g are
@@ -342,7 +423,7 @@ DI void compute_step(Ctx<kLds>& c, Dir& d, int s, int slot) {
   const uint32_t tl = (uint32_t)d.tlen, pl = (uint32_t)d.plen;
   int fm = INT_MAX, lm = INT_MIN, fi = INT_MAX, li = INT_MIN, fd = INT_MAX, ld = INT_MIN;
   int32_t* const ws = c.ws;
-  for (int kb = lo; kb <= hi; kb += kWave) {
+  for (int kb = lo + c.wid * kWave; kb <= hi; kb += K::kNW * kWave) {
     const int k = kb + c.lane;
     const bool act = k <= hi;
     // wavefront_compute_affine_idm (wavefront_compute_affine.c:65-85)
@@ -364,6 +445,11 @@ DI void compute_step(Ctx<kLds>& c, Dir& d, int s, int slot) {
     if (b_i) { if (fi == INT_MAX) fi = kb + __builtin_ctzll(b_i); li = kb + 63 - __builtin_clzll(b_i); }
     if (b_d) { if (fd == INT_MAX) fd = kb + __builtin_ctzll(b_d); ld = kb + 63 - __builtin_clzll(b_d); }
   }
+  if constexpr (K::kNW > 1) {   // first / last in-bounds diagonal over the team's waves
+    int nfm = -fm, nfi = -fi, nfd = -fd;
+    team_max6(c, nfm, lm, nfi, li, nfd, ld);
+    fm = -nfm; fi = -nfi; fd = -nfd;
+  }
   // trimmed limits: hi = last in-bounds (else lo-1), lo = first in-bounds (else lo)
   if (lm == INT_MIN) { fm = lo; lm = lo - 1; }
   if (li == INT_MIN) { fi = lo; li = lo - 1; }
@@ -371,14 +457,14 @@ DI void compute_step(Ctx<kLds>& c, Dir& d, int s, int slot) {
   set_hdr(c, d, s, slot, CM, fm, lm, bm);
   set_hdr(c, d, s, slot, CI, has_i ? fi : kAbsent, li, bi);
   set_hdr(c, d, s, slot, CD, has_d ? fd : kAbsent, ld, bd);
-  wave_sync<kLds>();
+  if constexpr (K::kNW == 1) wave_sync<K::kLds>();
 }
 
 // ---------------------------------------------------------------------------
 // extend + termination (wavefront_extend.c:86-211, wavefront_termination.c:37-114)
 
-template <bool kLds>
-DI bool terminated(const Ctx<kLds>& c, const Dir& d, int s, int slot) {
+template <class K>
+DI bool terminated(const Ctx<K>& c, const Dir& d, int s, int slot) {
   const int k_end = d.tlen - d.plen;
   int present;
   const Wf w = get_wf(c, d, s, slot, d.ce, &present);
@@ -387,8 +473,8 @@ DI bool terminated(const Ctx<kLds>& c, const Dir& d, int s, int slot) {
 }
 
 // Returns 1 when the aligner is done; *max_ak gets the max antidiagonal.
-template <bool kLds>
-DI int extend_step(Ctx<kLds>& c, Dir& d, int s, int slot, int* max_ak) {
+template <class K>
+DI int extend_step(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   *max_ak = 0;
   int present;
   const Wf m = get_wf(c, d, s, slot, CM, &present);
@@ -401,7 +487,7 @@ DI int extend_step(Ctx<kLds>& c, Dir& d, int s, int slot, int* max_ak) {
     return 0;
   }
   int lmax = 0;
-  for (int kb = m.lo; kb <= m.hi; kb += kWave) {
+  for (int kb = m.lo + c.wid * kWave; kb <= m.hi; kb += K::kNW * kWave) {
     const int k = kb + c.lane;
     if (k <= m.hi) {
       const int a = m.base + (k & m.mask);
@@ -413,8 +499,12 @@ DI int extend_step(Ctx<kLds>& c, Dir& d, int s, int slot, int* max_ak) {
       }
     }
   }
-  wave_sync<kLds>();
-  lmax = wave_max_nonneg(lmax);
+  if constexpr (K::kNW == 1) {
+    wave_sync<K::kLds>();
+    lmax = wave_max_nonneg(lmax);
+  } else {
+    lmax = team_max1(c, wave_max_nonneg(lmax));
+  }
   if (terminated(c, d, s, slot)) {
     d.status = kStEndReached;
     d.status_score = s;
@@ -429,8 +519,8 @@ DI int extend_step(Ctx<kLds>& c, Dir& d, int s, int slot, int* max_ak) {
 
 // First k0 (ascending) of the overlapping interval that satisfies the overlap
 // condition; component `comp` (CD/CI: indel-to-indel, CM: M-to-M).
-template <bool kLds>
-DI void scan_breakpoint(const Ctx<kLds>& c, bool forward, int s0, int s1, const Wf& w0,
+template <class K>
+DI void scan_breakpoint(const Ctx<K>& c, bool forward, int s0, int s1, const Wf& w0,
                         const Wf& w1, int comp, int tl, int pl, BP& bp) {
   const int lo0 = w0.lo, hi0 = w0.hi;
   const int lo1 = tl - pl - w1.hi, hi1 = tl - pl - w1.lo;
@@ -468,8 +558,51 @@ DI void scan_breakpoint(const Ctx<kLds>& c, bool forward, int s0, int s1, const 
   }
 }
 
-template <bool kLds>
-DI void overlap(const Ctx<kLds>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int s1,
+// Team form of scan_breakpoint: the first hit among the chunks this wave owns
+// (INT_MAX if none).  The team's answer is the minimum over its waves.
+template <class K>
+DI int scan_first(const Ctx<K>& c, bool forward, const Wf& w0, const Wf& w1, int comp, int tl, int pl) {
+  const int lo1 = tl - pl - w1.hi, hi1 = tl - pl - w1.lo;
+  if (hi1 < w0.lo || w0.hi < lo1) return INT_MAX;
+  const int min_hi = imin(w0.hi, hi1), max_lo = imax(w0.lo, lo1);
+  for (int kb = max_lo + c.wid * kWave; kb <= min_hi; kb += K::kNW * kWave) {
+    const int k0 = kb + c.lane;
+    const int k1 = tl - pl - k0;
+    bool ok = false;
+    if (k0 <= min_hi) {
+      const int o0 = c.ws[w0.base + (k0 & w0.mask)];
+      const int o1 = c.ws[w1.base + (k1 & w1.mask)];
+      ok = o0 + o1 >= tl;
+      if (ok && comp != CM) ok = forward ? !(o0 - k0 > pl || o0 > tl) : !(o1 - k1 > pl || o1 > tl);
+    }
+    const uint64_t b = __ballot(ok);
+    if (b) return kb + __builtin_ctzll(b);
+  }
+  return INT_MAX;
+}
+
+// Records the team's hit (slot j of `buf`) if it improves the breakpoint.
+template <class K>
+DI void accept_hit(const Ctx<K>& c, const int32_t* buf, int j, bool forward, int s0, int s1,
+                   const Wf& w0, const Wf& w1, int comp, int tl, int pl, BP& bp) {
+  const int sc = (comp == CM) ? s0 + s1 : s0 + s1 - c.o;
+  if (!(sc < bp.score)) return;
+  const int kk0 = team_min_slot(c, buf, j);
+  if (kk0 == INT_MAX) return;
+  const int kk1 = tl - pl - kk0;
+  const int a0 = uni(c.ws[w0.base + (kk0 & w0.mask)]);
+  const int a1 = uni(c.ws[w1.base + (kk1 & w1.mask)]);
+  if (forward) {
+    bp.sf = s0; bp.sr = s1; bp.kf = kk0; bp.kr = kk1; bp.of = a0; bp.orr = a1;
+  } else {
+    bp.sf = s1; bp.sr = s0; bp.kf = kk1; bp.kr = kk0; bp.of = a1; bp.orr = a0;
+  }
+  bp.score = sc;
+  bp.comp = comp;
+}
+
+template <class K>
+DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int s1,
                 int slot1, bool forward, BP& bp) {
   int pm0, pi0, pd0;
   const Wf m0 = get_wf(c, a0, s0, slot0, CM, &pm0);
@@ -477,25 +610,75 @@ DI void overlap(const Ctx<kLds>& c, const Dir& a0, const Dir& a1, int s0, int sl
   const Wf i0 = get_wf(c, a0, s0, slot0, CI, &pi0);
   const Wf d0 = get_wf(c, a0, s0, slot0, CD, &pd0);
   const int tl = a0.tlen, pl = a0.plen;
-  for (int i = 0; i < c.scope; ++i) {
-    const int si = s1 - i;
-    if (si < 0) break;
-    if (s0 + si - c.o >= bp.score) continue;
-    const int sl = slot_back(slot1, i, c.scope);
-    int p1;
-    const Wf d1 = get_wf(c, a1, si, sl, CD, &p1);
-    if (pd0 && p1) scan_breakpoint(c, forward, s0, si, d0, d1, CD, tl, pl, bp);
-    const Wf i1 = get_wf(c, a1, si, sl, CI, &p1);
-    if (pi0 && p1) scan_breakpoint(c, forward, s0, si, i0, i1, CI, tl, pl, bp);
-    if (s0 + si >= bp.score) continue;
-    const Wf m1 = get_wf(c, a1, si, sl, CM, &p1);
-    if (p1) scan_breakpoint(c, forward, s0, si, m0, m1, CM, tl, pl, bp);
+  if constexpr (K::kNW == 1) {
+    for (int i = 0; i < c.scope; ++i) {
+      const int si = s1 - i;
+      if (si < 0) break;
+      if (s0 + si - c.o >= bp.score) continue;
+      const int sl = slot_back(slot1, i, c.scope);
+      int p1;
+      const Wf d1 = get_wf(c, a1, si, sl, CD, &p1);
+      if (pd0 && p1) scan_breakpoint(c, forward, s0, si, d0, d1, CD, tl, pl, bp);
+      const Wf i1 = get_wf(c, a1, si, sl, CI, &p1);
+      if (pi0 && p1) scan_breakpoint(c, forward, s0, si, i0, i1, CI, tl, pl, bp);
+      if (s0 + si >= bp.score) continue;
+      const Wf m1 = get_wf(c, a1, si, sl, CM, &p1);
+      if (p1) scan_breakpoint(c, forward, s0, si, m0, m1, CM, tl, pl, bp);
+    }
+  } else {
+    // Every scan the sequential order could still take (judged by the score the
+    // call starts with) runs at once, each wave over its chunks; one barrier
+    // later the waves replay the sequential order on the team minima.  A scan's
+    // result does not depend on earlier acceptances, only whether it is taken.
+    int32_t* buf = team_buf(c);
+    int32_t* mine = buf + c.wid * kRedSlots;
+    const int entry = bp.score;
+    for (int i = 0; i < c.scope; ++i) {
+      const int si = s1 - i;
+      int hd = INT_MAX, hi = INT_MAX, hm = INT_MAX;
+      if (si >= 0 && s0 + si - c.o < entry) {
+        const int sl = slot_back(slot1, i, c.scope);
+        int p1;
+        const Wf d1 = get_wf(c, a1, si, sl, CD, &p1);
+        if (pd0 && p1) hd = scan_first(c, forward, d0, d1, CD, tl, pl);
+        const Wf i1 = get_wf(c, a1, si, sl, CI, &p1);
+        if (pi0 && p1) hi = scan_first(c, forward, i0, i1, CI, tl, pl);
+        if (s0 + si < entry) {
+          const Wf m1 = get_wf(c, a1, si, sl, CM, &p1);
+          if (p1) hm = scan_first(c, forward, m0, m1, CM, tl, pl);
+        }
+      }
+      if (c.lane == 0) {
+        mine[3 * i] = hd;
+        mine[3 * i + 1] = hi;
+        mine[3 * i + 2] = hm;
+      }
+    }
+    __syncthreads();
+    for (int i = 0; i < c.scope; ++i) {
+      const int si = s1 - i;
+      if (si < 0) break;
+      if (s0 + si - c.o >= bp.score) continue;
+      const int sl = slot_back(slot1, i, c.scope);
+      int p1;
+      const Wf d1 = get_wf(c, a1, si, sl, CD, &p1);
+      if (pd0 && p1) accept_hit(c, buf, 3 * i, forward, s0, si, d0, d1, CD, tl, pl, bp);
+      const Wf i1 = get_wf(c, a1, si, sl, CI, &p1);
+      if (pi0 && p1) accept_hit(c, buf, 3 * i + 1, forward, s0, si, i0, i1, CI, tl, pl, bp);
+      if (s0 + si >= bp.score) continue;
+      const Wf m1 = get_wf(c, a1, si, sl, CM, &p1);
+      if (p1) accept_hit(c, buf, 3 * i + 2, forward, s0, si, m0, m1, CM, tl, pl, bp);
+    }
+    // the winners' offsets are read before any wave moves the rings on
+    __syncthreads();
   }
 }
 
 // wavefront_aligner_init_wf (wavefront_aligner.c:311-383) for a ring aligner.
-template <bool kLds>
-DI void ring_init(Ctx<kLds>& c, Dir& d) {
+template <class K>
+DI void ring_init(Ctx<K>& c, Dir& d) {
+  // a team's slower waves may still read the previous sub-problem's workspace
+  if constexpr (K::kNW > 1) __syncthreads();
   const uint64_t keep = ~(0xffffffffull << (32 * d.ring));
   for (int q = 0; q < 3; ++q) c.rpres[q] &= keep;
   set_hdr(c, d, 0, 0, d.cb, 0, 0, 0);
@@ -503,14 +686,14 @@ DI void ring_init(Ctx<kLds>& c, Dir& d) {
   d.null_steps = 0;
   d.status = kStOk;
   d.status_score = 0;
-  wave_sync<kLds>();
+  team_sync<K>();
 }
 
 DI int next_slot(int slot, int scope) { return slot + 1 == scope ? 0 : slot + 1; }
 
 // wavefront_bialign_find_breakpoint (wavefront_bialign.c:411-519)
-template <bool kLds>
-DI int find_breakpoint(Ctx<kLds>& c, Dir& f, Dir& r, BP& bp) {
+template <class K>
+DI int find_breakpoint(Ctx<K>& c, Dir& f, Dir& r, BP& bp) {
   ring_init(c, f);
   ring_init(c, r);
   const int max_antidiagonal = f.plen + f.tlen - 1;
@@ -567,13 +750,13 @@ DI int find_breakpoint(Ctx<kLds>& c, Dir& f, Dir& r, BP& bp) {
 // ---------------------------------------------------------------------------
 // output runs
 
-template <bool kLds>
-DI void out_flush(Ctx<kLds>& c) {
+template <class K>
+DI void out_flush(Ctx<K>& c) {
   if (c.pend_len <= 0) return;
   if (c.out_n >= c.out_cap) {
     c.cap_err = 1;
   } else {
-    if (c.lane == 0) c.out[c.out_n] = ((uint32_t)c.pend_len << 4) | (uint32_t)c.pend_op;
+    if (c.leader) c.out[c.out_n] = ((uint32_t)c.pend_len << 4) | (uint32_t)c.pend_op;
     c.out_n++;
   }
   // cigar_score_gap_affine (cigar.c:277-303) over maximal runs
@@ -583,8 +766,8 @@ DI void out_flush(Ctx<kLds>& c) {
   c.pend_len = 0;
   c.pend_op = -1;
 }
-template <bool kLds>
-DI void out_append(Ctx<kLds>& c, int op, int n) {
+template <class K>
+DI void out_append(Ctx<K>& c, int op, int n) {
   if (n <= 0) return;
   if (op == c.pend_op) { c.pend_len += n; return; }
   out_flush(c);
@@ -596,13 +779,13 @@ DI void out_append(Ctx<kLds>& c, int op, int n) {
 struct RevRuns {
   int n, op, len;
 };
-template <bool kLds>
-DI void rev_emit(Ctx<kLds>& c, RevRuns& rr, int op, int n) {
+template <class K>
+DI void rev_emit(Ctx<K>& c, RevRuns& rr, int op, int n) {
   if (n <= 0) return;
   if (op == rr.op) { rr.len += n; return; }
   if (rr.len > 0) {
     if (rr.n >= c.L.rcap) { c.spill = 1; return; }
-    if (c.lane == 0) c.ws[c.L.runs_tmp + rr.n] = (int)(((uint32_t)rr.len << 4) | (uint32_t)rr.op);
+    if (c.lane == 0) c.pv[c.L.runs_tmp + rr.n] = (int)(((uint32_t)rr.len << 4) | (uint32_t)rr.op);
     rr.n++;
   }
   rr.op = op;
@@ -614,8 +797,8 @@ DI void rev_emit(Ctx<kLds>& c, RevRuns& rr, int op, int n) {
 
 constexpr int kBtM = 9, kBtD1Ext = 6, kBtD1Open = 5, kBtI1Ext = 2, kBtI1Open = 1;
 
-template <bool kLds>
-DI long long bt_src(const Ctx<kLds>& c, const Dir& b, int score, int comp, int k, int add, int type) {
+template <class K>
+DI long long bt_src(const Ctx<K>& c, const Dir& b, int score, int comp, int k, int add, int type) {
   int present;
   const Wf w = get_wf(c, b, score, 0, comp, &present);
   if (!present || !(w.lo <= k && k <= w.hi)) return (long long)kOffsetNull;
@@ -625,8 +808,8 @@ DI long long max64(long long a, long long b) { return a >= b ? a : b; }
 
 // wavefront_backtrace_affine (wavefront_backtrace.c:320-529); appends the
 // operations to the output in forward order.  Returns 0 or -1 on a trace error.
-template <bool kLds>
-DI int backtrace(Ctx<kLds>& c, const Dir& b, int score) {
+template <class K>
+DI int backtrace(Ctx<K>& c, const Dir& b, int score) {
   int matrix = b.ce;
   int k = b.tlen - b.plen, off = b.tlen;
   int h = off, v = off - k;
@@ -688,11 +871,11 @@ DI int backtrace(Ctx<kLds>& c, const Dir& b, int score) {
     return -1;
   }
   if (c.spill) return 0;
-  wave_sync<kLds>();
+  wave_sync<K::kLds>();
   // forward order: the pending run first, then the stored runs backwards
   out_append(c, rr.op, rr.len);
   for (int i = rr.n - 1; i >= 0; --i) {
-    const uint32_t run = (uint32_t)uni(c.ws[c.L.runs_tmp + i]);
+    const uint32_t run = (uint32_t)uni(c.pv[c.L.runs_tmp + i]);
     out_append(c, (int)(run & 15u), (int)(run >> 4));
   }
   return 0;
@@ -700,16 +883,18 @@ DI int backtrace(Ctx<kLds>& c, const Dir& b, int score) {
 
 // Unidirectional WFA over the sub-problem with all wavefronts kept
 // (wavefront_unialign.c:242-273).  Returns kStOk or kStUnattainable.
-template <bool kLds>
-DI int base_case(Ctx<kLds>& c, int pb, int pe, int tb, int te, int cb, int ce, int* score_out) {
+template <class K>
+DI int base_case(Ctx<K>& c, int pb, int pe, int tb, int te, int cb, int ce, int* score_out) {
   Dir b{pb, pe - pb, tb, te - tb, 0, -1, cb, ce, 0, kStOk, 0};
+  // a team's slower waves may still read the previous sub-problem's workspace
+  if constexpr (K::kNW > 1) __syncthreads();
   // score 0: the begin component at k=0 (wavefront_aligner.c:311-383)
   c.arena_top = 1;
   set_hdr(c, b, 0, 0, CM, cb == CM ? 0 : kAbsent, 0, c.L.data);
   set_hdr(c, b, 0, 0, CI, cb == CI ? 0 : kAbsent, 0, c.L.data);
   set_hdr(c, b, 0, 0, CD, cb == CD ? 0 : kAbsent, 0, c.L.data);
   c.ws[c.L.data] = 0;
-  wave_sync<kLds>();
+  team_sync<K>();
   int score = 0, ak;
   const int guard = score_guard(c, b.plen, b.tlen);
   for (;;) {
@@ -733,24 +918,24 @@ DI int base_case(Ctx<kLds>& c, int pb, int pe, int tb, int te, int cb, int ce, i
 struct Sub {
   int pb, pe, tb, te, cb, ce, rem, lvl;
 };
-template <bool kLds>
-DI void push(Ctx<kLds>& c, int& sp, const Sub& s) {
+template <class K>
+DI void push(Ctx<K>& c, int& sp, const Sub& s) {
   if (sp >= c.L.scap) { c.spill = 1; return; }
-  int* e = c.ws + c.L.stack + sp * 8;
+  int* e = c.pv + c.L.stack + sp * 8;
   e[0] = s.pb; e[1] = s.pe; e[2] = s.tb; e[3] = s.te;
   e[4] = s.cb; e[5] = s.ce; e[6] = s.rem; e[7] = s.lvl;
   ++sp;
 }
-template <bool kLds>
-DI Sub pop(Ctx<kLds>& c, int& sp) {
+template <class K>
+DI Sub pop(Ctx<K>& c, int& sp) {
   --sp;
-  const int* e = c.ws + c.L.stack + sp * 8;
+  const int* e = c.pv + c.L.stack + sp * 8;
   return Sub{uni(e[0]), uni(e[1]), uni(e[2]), uni(e[3]), uni(e[4]), uni(e[5]), uni(e[6]), uni(e[7])};
 }
 
 // Aligns the staged pair.  Returns the final WFA status (wavefront_bialign.c:722-729).
-template <bool kLds>
-DI int align_pair(Ctx<kLds>& c, int plen, int tlen, int score_only, int* wf_score) {
+template <class K>
+DI int align_pair(Ctx<K>& c, int plen, int tlen, int score_only, int* wf_score) {
   *wf_score = INT_MIN;
   if (score_only) {
     // wavefront_bialign_compute_score (wavefront_bialign.c:662-699)
@@ -770,7 +955,7 @@ DI int align_pair(Ctx<kLds>& c, int plen, int tlen, int score_only, int* wf_scor
   int sp = 0;
   const bool min_length = imax(plen, tlen) <= kFallbackMinLength;
   push(c, sp, Sub{0, plen, 0, tlen, CM, CM, min_length ? 0 : INT_MAX, 0});
-  wave_sync<kLds>();
+  wave_sync<K::kLds>();
   int status = kStOk;
   int iters = 0;
   const int max_iters = 4 * (plen + tlen) + 64;
@@ -814,7 +999,7 @@ DI int align_pair(Ctx<kLds>& c, int plen, int tlen, int score_only, int* wf_scor
     push(c, sp, Sub{s.pb + bv, s.pe, s.tb + bh, s.te, bp.comp, s.ce, bp.sr, s.lvl + 1});
     push(c, sp, Sub{s.pb, s.pb + bv, s.tb, s.tb + bh, s.cb, bp.comp, bp.sf, s.lvl + 1});
     if (c.spill) return kStCapacity;
-    wave_sync<kLds>();
+    wave_sync<K::kLds>();
   }
   out_flush(c);
   if (status == kStOk) return 0;
@@ -825,19 +1010,37 @@ DI int align_pair(Ctx<kLds>& c, int plen, int tlen, int score_only, int* wf_scor
 // ---------------------------------------------------------------------------
 // kernels
 
-template <bool kLds>
-__global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
+template <class K>
+__global__ __launch_bounds__(K::kNW > 1 ? K::kNW * kWave : kWave * kWavesPerBlock)
+void biwfa_persistent(KParams p) {
   extern __shared__ int32_t lds[];
+  __shared__ int32_t tred[K::kNW > 1 ? 2 * K::kNW * kRedSlots : 1];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
-  const int waves_per_block = blockDim.x / kWave;
-  int32_t* ws = kLds ? lds + wv * p.L.total
-                     : p.gws + (int64_t)(blockIdx.x * waves_per_block + wv) * p.gws_stride;
+  const int wid = K::kNW > 1 ? wv : 0;
+  const int64_t team = K::kNW > 1 ? (int64_t)blockIdx.x : (int64_t)blockIdx.x * (blockDim.x / kWave) + wv;
+  int32_t* ws = K::kLds ? lds + wv * p.L.total : p.gws + team * p.gws_stride;
+  Ctx<K> c;
+  c.ws = ws;
+  c.pv = ws + p.L.priv + wid * p.L.priv_ints;
+  c.lane = lane;
+  c.wid = wid;
+  c.leader = wid == 0 && lane == 0;
+  c.tred = tred;
+  c.tph = 0;
+  c.L = p.L;
+  c.x = p.x; c.o = p.o; c.e = p.e; c.scope = p.scope;
+  c.max_steps = p.max_steps;
+  c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
+  c.gmask = p.gmask;
   const int count = uni(*(volatile const int32_t*)p.count);
   for (;;) {
     int base = 0;
-    if (lane == 0) base = atomicAdd(p.queue, p.chunk);
+    if (c.leader) base = atomicAdd(p.queue, p.chunk);
     base = uni(__shfl(base, 0, kWave));
+    if constexpr (K::kNW > 1) {   // wave 0's value to the whole team
+      base = team_max1(c, wid == 0 ? base : INT_MIN);
+    }
     if (base >= count) break;
     const int end = imin(base + p.chunk, count);
     for (int w = base; w < end; ++w) {
@@ -845,28 +1048,20 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
       const int plen = uni(p.p_len[pair]), tlen = uni(p.t_len[pair]);
       const uint64_t* gP = p.words + p.p_woff[pair];
       const uint64_t* gT = p.words + p.t_woff[pair];
-      Ctx<kLds> c;
-      c.ws = ws;
-      c.lane = lane;
-      c.L = p.L;
-      c.x = p.x; c.o = p.o; c.e = p.e; c.scope = p.scope;
-      c.max_steps = p.max_steps;
-      c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
       c.arena_top = 0;
       for (int q = 0; q < 3; ++q) {
         c.rlo[q] = kAbsent; c.rhi[q] = 0;
         c.blo[q] = kAbsent; c.bhi[q] = 0; c.bkb[q] = 0;
         c.rpres[q] = 0; c.bpres[q] = 0;
       }
-      c.gmask = p.gmask;
       c.out = p.runs + p.runs_off[pair];
       c.out_cap = (int)(p.runs_off[pair + 1] - p.runs_off[pair]);
       c.out_n = 0; c.pend_op = -1; c.pend_len = 0; c.alg_score = 0;
       c.spill = 0; c.cap_err = 0; c.guard = 0;
-      if (kLds) {
+      const int pw = (plen + 31) >> 5, tw = (tlen + 31) >> 5;
+      if (K::kLds) {
         // stage both sequences: [pad][words...][pad][pad] (the pads keep the
         // 64-bit windows in bounds at either end)
-        const int pw = (plen + 31) >> 5, tw = (tlen + 31) >> 5;
         if (pw + 3 > p.L.seq_words || tw + 3 > p.L.seq_words) {
           c.spill = 1;
         } else {
@@ -880,17 +1075,35 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
           }
           c.P = sP + 1;
           c.T = sT + 1;
-          wave_sync<kLds>();
+          wave_sync<K::kLds>();
         }
       } else {
         c.P = gP;
         c.T = gT;
+        if constexpr (K::kNW > 1) {
+          // a team stages the pair in LDS when it fits (same padded layout)
+          const int sw = p.seq_lds_words;
+          if (pw + 3 <= sw && tw + 3 <= sw) {
+            uint64_t* sP = reinterpret_cast<uint64_t*>(lds);
+            uint64_t* sT = sP + sw;
+            __syncthreads();   // the previous pair's readers are done
+            for (int i = threadIdx.x; i < pw + 3; i += K::kNW * kWave) {
+              sP[i] = (i >= 1 && i <= pw) ? gP[i - 1] : 0ull;
+            }
+            for (int i = threadIdx.x; i < tw + 3; i += K::kNW * kWave) {
+              sT[i] = (i >= 1 && i <= tw) ? gT[i - 1] : 0ull;
+            }
+            __syncthreads();
+            c.P = sP + 1;
+            c.T = sT + 1;
+          }
+        }
       }
       int wf = INT_MIN, st = kStCapacity;
       if (!c.spill) st = align_pair(c, plen, tlen, p.score_only, &wf);
       if (c.spill) {
         if (p.next_list) {      // re-run by the next tier with larger capacities
-          if (lane == 0) {
+          if (c.leader) {
             const int slot = atomicAdd(p.next_count, 1);
             p.next_list[slot] = pair;
           }
@@ -899,7 +1112,7 @@ __global__ __launch_bounds__(256) void biwfa_persistent(KParams p) {
         st = kStCapacity;
       }
       if (c.cap_err) st = kStCapacity;
-      if (lane == 0) {
+      if (c.leader) {
         p.status[pair] = st;
         p.wf_score[pair] = wf;
         p.score[pair] = (p.score_only || c.out_n > 0) ? c.alg_score : -1;
@@ -929,7 +1142,11 @@ __global__ __launch_bounds__(256) void biwfa_compact(const uint32_t* runs, const
   for (int i = lane; i < cnt; i += kWave) dst[i] = src[i];
 }
 
-template __global__ void biwfa_persistent<true>(KParams);
-template __global__ void biwfa_persistent<false>(KParams);
+template __global__ void biwfa_persistent<Cfg<true, 1>>(KParams);
+template __global__ void biwfa_persistent<Cfg<false, 1>>(KParams);
+template __global__ void biwfa_persistent<Cfg<false, 2>>(KParams);
+template __global__ void biwfa_persistent<Cfg<false, 4>>(KParams);
+template __global__ void biwfa_persistent<Cfg<false, 8>>(KParams);
+template __global__ void biwfa_persistent<Cfg<false, 16>>(KParams);
 
 }  // namespace bw

@@ -21,7 +21,7 @@
 #include "biwfa_mi355x.h"
 
 namespace bw {
-template <bool kLds>
+template <class K>
 __global__ void biwfa_persistent(KParams p);
 __global__ void biwfa_reset(int32_t* counters, const int32_t* init, int n);
 __global__ void biwfa_compact(const uint32_t* runs, const int64_t* runs_off, const int32_t* nruns,
@@ -61,29 +61,34 @@ int next_pow2(int v) {
   return p;
 }
 
-// Per-wave workspace layout (int32 units).  Wavefront limits live in registers;
-// only the global-workspace kernel keeps base-case headers in memory
-// (mem_base_hdr), because its base cases may exceed one register lane per score.
+// Workspace layout of one aligner (int32 units): the shared ring / arena, then
+// one private region per wave of the team.  Wavefront limits live in
+// registers; only the global-workspace kernels keep base-case headers in
+// memory (mem_base_hdr), because their base cases may exceed one register lane
+// per score.
 bw::WsLayout make_layout(int scope, int wcap, int arena_min, int hcap, int rcap, int scap,
-                         int seq_words, bool mem_base_hdr) {
+                         int seq_words, bool mem_base_hdr, int team_waves) {
   bw::WsLayout L{};
   int off = 0;
-  L.ring_hdr = 0;
   L.data = off;
   L.wcap = wcap;
   const int ring_ints = 2 * scope * 3 * wcap;
   L.arena_cap = std::max(ring_ints, arena_min);
   off += L.arena_cap;
-  L.bhdr = off;
+  off = (off + 15) & ~15;
+  L.priv = off;
+  int po = 0;
+  L.bhdr = po;
   L.hcap = hcap;
-  if (mem_base_hdr) off += hcap * 3 * 3;
-  L.runs_tmp = off;
+  if (mem_base_hdr) po += hcap * 3 * 3;
+  L.runs_tmp = po;
   L.rcap = rcap;
-  off += rcap;
-  L.stack = off;
+  po += rcap;
+  L.stack = po;
   L.scap = scap;
-  off += scap * 8;
-  off = (off + 3) & ~3;
+  po += scap * 8;
+  L.priv_ints = (po + 15) & ~15;
+  off += team_waves * L.priv_ints;
   L.seq = off;
   L.seq_words = seq_words;
   off += seq_words > 0 ? 2 * seq_words * 2 : 0;
@@ -131,12 +136,27 @@ int internal_penalties(const bw_affine_penalties_t& p, Penalties* out) {
 struct Tier {
   bool lds = false;
   bw::WsLayout L{};
+  int nw = 1;                // waves per aligner (team size)
   int waves_per_block = bw::kWavesPerBlock;
   int blocks = 0;
-  int64_t stride = 0;        // global tiers: ints per wave
+  int64_t aligners = 0;      // global tiers: workspace slabs in use
+  int64_t stride = 0;        // global tiers: ints per aligner
   int chunk = 1;
   int seq_cap = 0;           // LDS tier: longest sequence it can stage
+  int seq_lds_words = 0;     // team tiers: LDS words per staged sequence
+  size_t dyn_lds = 0;        // dynamic LDS bytes per workgroup
 };
+
+// Kernel instantiation of a global tier with `nw` waves per aligner.
+static const void* team_kernel(int nw) {
+  switch (nw) {
+    case 1: return reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<false, 1>>);
+    case 2: return reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<false, 2>>);
+    case 4: return reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<false, 4>>);
+    case 8: return reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<false, 8>>);
+    default: return reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<false, 16>>);
+  }
+}
 
 struct bw_dbatch {
   int64_t n = 0, nwords = 0;
@@ -294,6 +314,17 @@ static int base_score_bound(const Penalties& q) {
   return std::max(bw::kRecoveryMinScore, 100 * q.x + q.o + 100 * q.e) + 1;
 }
 
+// Waves per aligner of the global tier with ring width `wcap`: one wave while
+// a wavefront spans a few 64-diagonal chunks, then teams sized so that each
+// wave keeps about 2-4 chunks per step.
+static int team_waves(const bw_aligner_t* a, int wcap) {
+  (void)a;
+  if (wcap <= 256) return 1;
+  if (wcap <= 1024) return 4;
+  if (wcap <= 4096) return 8;
+  return 16;
+}
+
 static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b) {
   std::vector<Tier> tiers;
   const Penalties& q = a->pen;
@@ -306,14 +337,14 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
     t.lds = true;
     t.seq_cap = 512;
     const int seq_words = t.seq_cap / 32 + 3;
-    t.L = make_layout(scope, 32, 0, bw::kWave, 128, 16, seq_words, false);
+    t.L = make_layout(scope, 32, 0, bw::kWave, 128, 16, seq_words, false, 1);
     t.waves_per_block = bw::kWavesPerBlock;
-    const size_t block_bytes = (size_t)t.L.total * 4 * t.waves_per_block;
+    t.dyn_lds = (size_t)t.L.total * 4 * t.waves_per_block;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bw::biwfa_persistent<true>,
-                                                     bw::kWave * t.waves_per_block, block_bytes) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bw::biwfa_persistent<bw::Cfg<true, 1>>,
+                                                     bw::kWave * t.waves_per_block, t.dyn_lds) != hipSuccess ||
         nb <= 0)
-      nb = std::max<int>(1, (int)((160 * 1024) / block_bytes));
+      nb = std::max<int>(1, (int)((160 * 1024) / t.dyn_lds));
     if (a->attr.waves_per_cu > 0) nb = std::max(1, std::min(nb, a->attr.waves_per_cu / t.waves_per_block));
     t.blocks = a->cu_count * nb;
     t.chunk = a->attr.queue_chunk > 0 ? a->attr.queue_chunk : 4;
@@ -322,21 +353,45 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
   // global tiers: ring width 256, 1024, ... up to the batch maximum
   const int wmax = next_pow2(std::max(b->max_pt + 8, 256));
   const int64_t budget = (int64_t)24 << 30;
+  const int max_seq_words = (std::max(b->max_p, b->max_t) + 31) / 32 + 3;
   for (int w = 256;; w *= 4) {
     const bool last = w >= wmax || (int)tiers.size() + 1 >= kMaxTiers;
     const int wcap = last ? wmax : w;
     Tier t;
     t.lds = false;
+    t.nw = team_waves(a, wcap);
     const int width = std::min(2 * smax / q.e + 3, b->max_pt + 3);
     const int64_t worst = (int64_t)3 * ((smax + g) / g) * width;
     const int64_t arena = last ? worst : std::min<int64_t>(worst, (int64_t)3 * ((smax + g) / g) * std::min(wcap, width));
     const int steps = std::min(b->max_pt, smax / std::max(1, std::min(q.x, q.e)) + 1);
     t.L = make_layout(scope, wcap, (int)std::min<int64_t>(arena, (int64_t)1 << 30), smax + 2,
-                      2 * steps + 16, 64, 0, true);
+                      2 * steps + 16, 64, 0, true, t.nw);
     t.stride = t.L.total;
-    const int64_t per_wave = t.stride * 4;
-    int64_t waves = std::max<int64_t>(bw::kWavesPerBlock, std::min<int64_t>(budget / per_wave, (int64_t)a->cu_count * 8));
-    t.blocks = (int)std::max<int64_t>(1, waves / bw::kWavesPerBlock);
+    const int64_t per_aligner = t.stride * 4;
+    const void* kern = team_kernel(t.nw);
+    if (t.nw == 1) {
+      t.waves_per_block = bw::kWavesPerBlock;
+      const int64_t waves = std::max<int64_t>(bw::kWavesPerBlock,
+                                              std::min<int64_t>(budget / per_aligner, (int64_t)a->cu_count * 8));
+      t.blocks = (int)std::max<int64_t>(1, waves / bw::kWavesPerBlock);
+      t.aligners = (int64_t)t.blocks * bw::kWavesPerBlock;
+    } else {
+      // one team per workgroup; the pair is staged in LDS when the batch's
+      // longest sequence fits the team's share of the CU's LDS
+      t.waves_per_block = t.nw;
+      const int teams_cu = std::max(1, 20 / t.nw);
+      const int64_t red_bytes = (int64_t)2 * t.nw * bw::kRedSlots * 4;
+      const int64_t share = (int64_t)(160 * 1024) / teams_cu - red_bytes;
+      t.seq_lds_words = (max_seq_words * 16 <= share) ? max_seq_words : 0;
+      t.dyn_lds = (size_t)t.seq_lds_words * 16;
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, bw::kWave * t.nw, t.dyn_lds) != hipSuccess ||
+          nb <= 0)
+        nb = 1;
+      const int64_t teams = std::max<int64_t>(1, std::min<int64_t>(budget / per_aligner, (int64_t)a->cu_count * nb));
+      t.blocks = (int)teams;
+      t.aligners = teams;
+    }
     t.chunk = 1;
     tiers.push_back(t);
     if (last || (int)tiers.size() >= kMaxTiers) break;
@@ -415,7 +470,7 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
   if ((e = dalloc(&b->d_runs_off, n + 1)) != hipSuccess) return fail(e);
   int64_t gbytes = 0;
   for (const Tier& t : b->tiers)
-    if (!t.lds) gbytes = std::max<int64_t>(gbytes, (int64_t)t.blocks * bw::kWavesPerBlock * t.stride * 4);
+    if (!t.lds) gbytes = std::max<int64_t>(gbytes, t.aligners * t.stride * 4);
   if ((e = hipMalloc(reinterpret_cast<void**>(&b->d_gws), std::max<int64_t>(gbytes, 64))) != hipSuccess) return fail(e);
   b->gws_bytes = gbytes;
   hipStream_t s = a->stream;
@@ -537,15 +592,15 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
     p.L = T.L;
     p.gws = b->d_gws;
     p.gws_stride = T.stride;
+    p.seq_lds_words = T.seq_lds_words;
     if (T.lds) {
-      const size_t lds = (size_t)T.L.total * 4 * T.waves_per_block;
-      hipLaunchKernelGGL(bw::biwfa_persistent<true>, dim3(T.blocks), dim3(bw::kWave * T.waves_per_block),
-                         lds, s, p);
+      bw::biwfa_persistent<bw::Cfg<true, 1>><<<dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), T.dyn_lds, s>>>(p);
+      HIP_TRY(hipGetLastError());
     } else {
-      hipLaunchKernelGGL(bw::biwfa_persistent<false>, dim3(T.blocks), dim3(bw::kWave * bw::kWavesPerBlock),
-                         0, s, p);
+      void* args[] = {&p};
+      HIP_TRY(hipLaunchKernel(team_kernel(T.nw), dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), args,
+                              T.dyn_lds, s));
     }
-    HIP_TRY(hipGetLastError());
     if (t == 0) HIP_TRY(hipEventRecord(a->ev[2], s));
   }
   HIP_TRY(hipEventRecord(a->ev[3], s));
