@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--pairs", type=int, default=None, help="first N pairs of the workload")
     ap.add_argument("--schedule", default="length", choices=["length", "input"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", type=int, default=0, metavar="N",
+                    help="tune launch shapes on an N-pair sample of the workload before timing")
+    ap.add_argument("--tuning", default=None, help="JSON launch tuning (bw_tuning_t fields)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -128,20 +131,29 @@ def main():
 
     sched = bw.SCHEDULE_LENGTH_SORTED if args.schedule == "length" else bw.SCHEDULE_INPUT_ORDER
     aligner = bw.WavefrontAligner(penalties=bw.configs.PENALTIES, schedule=sched)
+    if args.tuning:
+        aligner.tuning = json.loads(args.tuning)
+    if args.autotune > 0:
+        t0 = time.time()
+        sample = bw.generate_dataset(cfg, num_pairs=min(args.autotune, n), seed=cfg.seed + 1000).pack()
+        tuned = aligner.autotune(sample, repeats=2)
+        log("autotuned on %d pairs in %.1f s: %s" % (len(sample), time.time() - t0, json.dumps(tuned)))
     db = aligner.upload(packed)
     for w in range(args.warmup):
         db.align()
         log("warmup %d done (%.2f ms)" % (w, db.timing()[0]))
-    kern_main, kern_spill, kern_total = [], [], []
+    kern_main, kern_spill, kern_total, kern_tiers = [], [], [], []
     t_start = time.perf_counter()
     for k in range(args.steps):
         db.align()
         tot, main_ms, spill_ms = db.timing()
         kern_total.append(tot); kern_main.append(main_ms); kern_spill.append(spill_ms)
+        kern_tiers.append(aligner.tier_timing())
     t_end = time.perf_counter()
     ms_per_step = (t_end - t_start) * 1000.0 / args.steps
-    log("timed %d steps: %.3f ms/step (kernels %.3f ms: main %.3f + spill %.3f)" % (
-        args.steps, ms_per_step, np.mean(kern_total), np.mean(kern_main), np.mean(kern_spill)))
+    tier_ms = [round(float(v), 4) for v in np.mean(np.array(kern_tiers), axis=0)]
+    log("timed %d steps: %.3f ms/step (kernels %.3f ms: main %.3f + spill %.3f; tiers %s)" % (
+        args.steps, ms_per_step, np.mean(kern_total), np.mean(kern_main), np.mean(kern_spill), tier_ms))
     res = db.download()
     bad_status = int(np.sum(res.status != 0))
     nruns = np.diff(res.cigar_offset)
@@ -172,7 +184,8 @@ def main():
         },
         "gcups": round(float(np.sum(packed.p_len.astype(np.float64) * packed.t_len)) / (ms_per_step / 1000.0) / 1e9, 3),
         "kernel_ms": {"total": round(float(np.mean(kern_total)), 4), "main": round(float(np.mean(kern_main)), 4),
-                      "spill": round(float(np.mean(kern_spill)), 4)},
+                      "spill": round(float(np.mean(kern_spill)), 4), "tiers": tier_ms},
+        "tuning": aligner.tuning,
         "status_errors": bad_status,
         "roofline": {
             "bound": "hbm",

@@ -67,6 +67,14 @@ class _Result(ctypes.Structure):
                 ("cigar_total", ctypes.c_int64)]
 
 
+MAX_TIERS = 8
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("lds_ring_width", ctypes.c_int), ("lds_seq_cap", ctypes.c_int),
+                ("team_waves", ctypes.c_int * MAX_TIERS), ("queue_chunk", ctypes.c_int * MAX_TIERS)]
+
+
 class _DatasetParams(ctypes.Structure):
     _fields_ = [("num_pairs", ctypes.c_int64), ("seed", ctypes.c_uint64),
                 ("len_min", ctypes.c_int32), ("len_max", ctypes.c_int32),
@@ -112,6 +120,10 @@ def lib():
         L.bw_last_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.bw_set_schedule.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.bw_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Tuning)]
+        L.bw_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Tuning)]
+        L.bw_last_tier_timing.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.bw_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch), ctypes.c_int, ctypes.POINTER(_Tuning)]
         L.bw_dataset_lengths.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_generate.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]
@@ -413,6 +425,44 @@ class WavefrontAligner:
 
     def upload(self, packed):
         return DeviceBatch(self, packed)
+
+    # -- launch tuning (speed only; results never change) --
+    @property
+    def tuning(self):
+        """dict(lds_ring_width, lds_seq_cap, team_waves[8], queue_chunk[8]); zeros mean
+        the built-in rule.  Applies to batches uploaded after it is set."""
+        t = _Tuning()
+        self._L.bw_get_tuning(self._h, ctypes.byref(t))
+        return {"lds_ring_width": t.lds_ring_width, "lds_seq_cap": t.lds_seq_cap,
+                "team_waves": list(t.team_waves), "queue_chunk": list(t.queue_chunk)}
+
+    @tuning.setter
+    def tuning(self, d):
+        t = _Tuning()
+        t.lds_ring_width = int(d.get("lds_ring_width", 0))
+        t.lds_seq_cap = int(d.get("lds_seq_cap", 0))
+        for i, v in enumerate(list(d.get("team_waves", []))[:MAX_TIERS]):
+            t.team_waves[i] = int(v)
+        for i, v in enumerate(list(d.get("queue_chunk", []))[:MAX_TIERS]):
+            t.queue_chunk[i] = int(v)
+        st = self._L.bw_set_tuning(self._h, ctypes.byref(t))
+        if st != 0:
+            raise BiwfaError("bw_set_tuning: " + last_error())
+
+    def autotune(self, packed, repeats=2):
+        """Times the sample under candidate tunings (bw_autotune); keeps and returns the best."""
+        b = packed._struct()
+        t = _Tuning()
+        st = self._L.bw_autotune(self._h, ctypes.byref(b), int(repeats), ctypes.byref(t))
+        if st != 0:
+            raise BiwfaError("bw_autotune: %s (%s)" % (strerror(st), last_error()))
+        return self.tuning
+
+    def tier_timing(self):
+        """Per-tier milliseconds of the last device-batch alignment (LDS tier first)."""
+        ms = np.zeros(MAX_TIERS, np.float32)
+        n = self._L.bw_last_tier_timing(self._h, _ptr(ms), MAX_TIERS)
+        return ms[:max(0, n)].tolist()
 
 
 class DeviceBatch:

@@ -186,27 +186,28 @@ DI int32_t* team_buf(Ctx<K>& c) {
   return b;
 }
 
-// In-place max over the team of up to six wave-uniform values (one barrier).
+// In-place max over the team of eight wave-uniform values (one barrier).
 // Lane j of every wave publishes value j; after the barrier lane j folds the
 // waves' copies of value j.  The lane's value is picked with masking
 // arithmetic: a select chain would become a select of the arguments' addresses
 // and move them to scratch.
 template <class K>
-DI void team_max6(Ctx<K>& c, int& v0, int& v1, int& v2, int& v3, int& v4, int& v5) {
+DI void team_max8(Ctx<K>& c, int& v0, int& v1, int& v2, int& v3, int& v4, int& v5, int& v6, int& v7) {
   int32_t* b = team_buf(c);
   const int l = c.lane;
-  const int a0 = v0, a1 = v1, a2 = v2, a3 = v3, a4 = v4, a5 = v5;
+  const int a0 = v0, a1 = v1, a2 = v2, a3 = v3, a4 = v4, a5 = v5, a6 = v6, a7 = v7;
   const int mine = (a0 & -(int)(l == 0)) | (a1 & -(int)(l == 1)) | (a2 & -(int)(l == 2)) |
-                   (a3 & -(int)(l == 3)) | (a4 & -(int)(l == 4)) | (a5 & -(int)(l == 5));
-  if (l < 6) b[c.wid * kRedSlots + l] = mine;
+                   (a3 & -(int)(l == 3)) | (a4 & -(int)(l == 4)) | (a5 & -(int)(l == 5)) |
+                   (a6 & -(int)(l == 6)) | (a7 & -(int)(l == 7));
+  if (l < 8) b[c.wid * kRedSlots + l] = mine;
   __syncthreads();
   int r = INT_MIN;
-  if (l < 6) {
+  if (l < 8) {
 #pragma unroll
     for (int w = 0; w < K::kNW; ++w) r = imax(r, b[w * kRedSlots + l]);
   }
-  v0 = rdl(r, 0); v1 = rdl(r, 1); v2 = rdl(r, 2);
-  v3 = rdl(r, 3); v4 = rdl(r, 4); v5 = rdl(r, 5);
+  v0 = rdl(r, 0); v1 = rdl(r, 1); v2 = rdl(r, 2); v3 = rdl(r, 3);
+  v4 = rdl(r, 4); v5 = rdl(r, 5); v6 = rdl(r, 6); v7 = rdl(r, 7);
 }
 template <class K>
 DI int team_max1(Ctx<K>& c, int v) {
@@ -379,22 +380,42 @@ DI void null_step(Ctx<K>& c, Dir& d, int s, int slot) {
   set_hdr(c, d, s, slot, CD, kAbsent, 0, 0);
 }
 
+// One WFA step at score s: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
+// followed by wavefront_extend_end2end(_max) (wavefront_extend.c:86-211) and the
+// termination test (wavefront_termination.c:37-114), fused: the lane that
+// computes M[s][k] extends it straight away, so the new wavefront is written
+// once and the trims, the maximum antidiagonal and the end-diagonal offset are
+// combined in a single reduction.  Extension only reads M[s][k] and the
+// sequences, and the trimmed-away diagonals are NULL, so this is the same
+// computation as compute-all-then-extend.  Returns 1 when the aligner is done
+// (end reached, or END_UNREACHABLE after scope+1 null steps); *max_ak gets the
+// maximum antidiagonal 2*offset-k over the extended cells.
 template <class K>
-DI void compute_step(Ctx<K>& c, Dir& d, int s, int slot) {
+DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
+  *max_ak = 0;
   const int sc = c.scope;
-  if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return; }
+  if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return 0; }
+  bool null = (s & c.gmask) != 0;
   // With penalties that share a power-of-two factor g, only multiples of g are
   // reachable scores: the other steps are null without looking at the inputs.
-  if (s & c.gmask) { null_step(c, d, s, slot); return; }
-  const Wf mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
-  const Wf mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
-  const Wf ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
-  const Wf de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
-  const bool nmm = mm.lo > mm.hi, nmo = mo.lo > mo.hi, nie = ie.lo > ie.hi, nde = de.lo > de.hi;
-  if (nmm && nmo && nie && nde) {               // null step (wavefront_compute_affine.c:245-251)
-    null_step(c, d, s, slot);
-    return;
+  Wf mm, mo, ie, de;
+  if (!null) {
+    mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
+    mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
+    ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
+    de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
+    null = mm.lo > mm.hi && mo.lo > mo.hi && ie.lo > ie.hi && de.lo > de.hi;
   }
+  if (null) {                                   // null step (wavefront_compute_affine.c:245-251)
+    null_step(c, d, s, slot);
+    if (d.null_steps > c.scope) {               // nothing left to extend (wavefront_extend.c)
+      d.status = kStEndUnreachable;
+      d.status_score = s;
+      return 1;
+    }
+    return 0;
+  }
+  const bool nmo = mo.lo > mo.hi, nie = ie.lo > ie.hi, nde = de.lo > de.hi;
   d.null_steps = 0;
   // wavefront_compute_limits_input (wavefront_compute.c:40-86)
   int lo = mm.lo, hi = mm.hi;
@@ -405,14 +426,14 @@ DI void compute_step(Ctx<K>& c, Dir& d, int s, int slot) {
   const int width = hi - lo + 1;
   int bm, bi, bd, mask;
   if (d.ring >= 0) {
-    if (width > c.L.wcap) { c.spill = 1; return; }
+    if (width > c.L.wcap) { c.spill = 1; return 0; }
     bm = ring_base(c, d.ring, slot, CM);
     bi = ring_base(c, d.ring, slot, CI);
     bd = ring_base(c, d.ring, slot, CD);
     mask = c.L.wcap - 1;
   } else {
     const int need = width * (1 + (int)has_i + (int)has_d);
-    if (c.arena_top + need > c.L.arena_cap) { c.spill = 1; return; }
+    if (c.arena_top + need > c.L.arena_cap) { c.spill = 1; return 0; }
     int top = c.L.data + c.arena_top;
     bm = top - lo; top += width;
     bi = top - lo; if (has_i) top += width;
@@ -421,34 +442,66 @@ DI void compute_step(Ctx<K>& c, Dir& d, int s, int slot) {
     mask = -1;
   }
   const uint32_t tl = (uint32_t)d.tlen, pl = (uint32_t)d.plen;
+  const int k_end = d.tlen - d.plen;
+  const int ce = d.ce;
   int fm = INT_MAX, lm = INT_MIN, fi = INT_MAX, li = INT_MIN, fd = INT_MAX, ld = INT_MIN;
+  int lmax = 0, endv = INT_MIN;
   int32_t* const ws = c.ws;
-  for (int kb = lo + c.wid * kWave; kb <= hi; kb += K::kNW * kWave) {
-    const int k = kb + c.lane;
-    const bool act = k <= hi;
-    // wavefront_compute_affine_idm (wavefront_compute_affine.c:65-85)
-    const int ins1 = imax(rd(ws, mo, k - 1), rd(ws, ie, k - 1)) + 1;
-    const int del1 = imax(rd(ws, mo, k + 1), rd(ws, de, k + 1));
-    const int misms = rd(ws, mm, k) + 1;
-    int mx = imax(del1, imax(misms, ins1));
-    if ((uint32_t)mx > tl || (uint32_t)(mx - k) > pl) mx = kOffsetNull;
-    if (act) {
-      ws[bm + (k & mask)] = mx;
-      if (has_i) ws[bi + (k & mask)] = ins1;
-      if (has_d) ws[bd + (k & mask)] = del1;
+  // A wave takes kG of its chunks at a time and issues all their loads before
+  // any store: the stores of one chunk could alias the next chunk's inputs as
+  // far as the compiler knows, so without the grouping every chunk would wait
+  // out a full memory latency on its own.  (The LDS ring is fast enough and
+  // narrow enough for one chunk at a time.)
+  constexpr int kG = K::kLds ? 1 : 4;
+  constexpr int kStride = K::kNW * kWave;
+  for (int kb0 = lo + c.wid * kWave; kb0 <= hi; kb0 += kG * kStride) {
+    int ins1[kG], del1[kG], misms[kG];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+      const int k = kb0 + g * kStride + c.lane;
+      // wavefront_compute_affine_idm (wavefront_compute_affine.c:65-85)
+      ins1[g] = imax(rd(ws, mo, k - 1), rd(ws, ie, k - 1)) + 1;
+      del1[g] = imax(rd(ws, mo, k + 1), rd(ws, de, k + 1));
+      misms[g] = rd(ws, mm, k) + 1;
     }
-    // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
-    const uint64_t b_m = __ballot(act && mx != kOffsetNull);
-    const uint64_t b_i = __ballot(act && (uint32_t)ins1 <= tl && (uint32_t)(ins1 - k) <= pl);
-    const uint64_t b_d = __ballot(act && (uint32_t)del1 <= tl && (uint32_t)(del1 - k) <= pl);
-    if (b_m) { if (fm == INT_MAX) fm = kb + __builtin_ctzll(b_m); lm = kb + 63 - __builtin_clzll(b_m); }
-    if (b_i) { if (fi == INT_MAX) fi = kb + __builtin_ctzll(b_i); li = kb + 63 - __builtin_clzll(b_i); }
-    if (b_d) { if (fd == INT_MAX) fd = kb + __builtin_ctzll(b_d); ld = kb + 63 - __builtin_clzll(b_d); }
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+      const int kb = kb0 + g * kStride;
+      if (kb > hi) break;
+      const int k = kb + c.lane;
+      const bool act = k <= hi;
+      int mx = imax(del1[g], imax(misms[g], ins1[g]));
+      const bool live = act && !((uint32_t)mx > tl || (uint32_t)(mx - k) > pl);
+      if (live) {                                 // extension along the diagonal
+        mx += lcp(c, d, mx - k, mx);
+        lmax = imax(lmax, 2 * mx - k);
+      } else {
+        mx = kOffsetNull;
+      }
+      if (act) {
+        ws[bm + (k & mask)] = mx;
+        if (has_i) ws[bi + (k & mask)] = ins1[g];
+        if (has_d) ws[bd + (k & mask)] = del1[g];
+        if (k == k_end) endv = (ce == CM) ? mx : (ce == CI) ? ins1[g] : del1[g];
+      }
+      // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
+      const uint64_t b_m = __ballot(live);
+      const uint64_t b_i = __ballot(act && (uint32_t)ins1[g] <= tl && (uint32_t)(ins1[g] - k) <= pl);
+      const uint64_t b_d = __ballot(act && (uint32_t)del1[g] <= tl && (uint32_t)(del1[g] - k) <= pl);
+      if (b_m) { if (fm == INT_MAX) fm = kb + __builtin_ctzll(b_m); lm = kb + 63 - __builtin_clzll(b_m); }
+      if (b_i) { if (fi == INT_MAX) fi = kb + __builtin_ctzll(b_i); li = kb + 63 - __builtin_clzll(b_i); }
+      if (b_d) { if (fd == INT_MAX) fd = kb + __builtin_ctzll(b_d); ld = kb + 63 - __builtin_clzll(b_d); }
+    }
   }
-  if constexpr (K::kNW > 1) {   // first / last in-bounds diagonal over the team's waves
+  // offset at k_end: held by lane (k_end - lo) % 64 of the wave owning its chunk
+  int endu = (k_end >= lo && k_end <= hi) ? rdl(endv, (k_end - lo) & (kWave - 1)) : INT_MIN;
+  lmax = wave_max_nonneg(lmax);
+  if constexpr (K::kNW > 1) {   // combine the team's waves
     int nfm = -fm, nfi = -fi, nfd = -fd;
-    team_max6(c, nfm, lm, nfi, li, nfd, ld);
+    team_max8(c, nfm, lm, nfi, li, nfd, ld, lmax, endu);
     fm = -nfm; fi = -nfi; fd = -nfd;
+  } else {
+    wave_sync<K::kLds>();
   }
   // trimmed limits: hi = last in-bounds (else lo-1), lo = first in-bounds (else lo)
   if (lm == INT_MIN) { fm = lo; lm = lo - 1; }
@@ -457,7 +510,17 @@ DI void compute_step(Ctx<K>& c, Dir& d, int s, int slot) {
   set_hdr(c, d, s, slot, CM, fm, lm, bm);
   set_hdr(c, d, s, slot, CI, has_i ? fi : kAbsent, li, bi);
   set_hdr(c, d, s, slot, CD, has_d ? fd : kAbsent, ld, bd);
-  if constexpr (K::kNW == 1) wave_sync<K::kLds>();
+  // wavefront_termination_end2end: the end component holds k_end with offset >= tlen
+  const bool ce_present = ce == CM || (ce == CI ? has_i : has_d);
+  const int ce_lo = ce == CM ? fm : ce == CI ? fi : fd;
+  const int ce_hi = ce == CM ? lm : ce == CI ? li : ld;
+  if (ce_present && ce_lo <= k_end && k_end <= ce_hi && endu >= d.tlen) {
+    d.status = kStEndReached;
+    d.status_score = s;
+    return 1;
+  }
+  *max_ak = lmax;
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -707,17 +770,15 @@ DI int find_breakpoint(Ctx<K>& c, Dir& f, Dir& r, BP& bp) {
     if (f_max + r_max >= max_antidiagonal) break;
     if (sf + sr > c.guard) return kStGuard;
     ++sf; qf = next_slot(qf, c.scope);
-    compute_step(c, f, sf, qf);
+    const int fq = compute_extend(c, f, sf, qf, &ak);
     if (c.spill) return kStCapacity;
-    const int fq = extend_step(c, f, sf, qf, &ak);
     if (f_max < ak) f_max = ak;
     last_forward = true;
     if (fq) return f.status;
     if (f_max + r_max >= max_antidiagonal) break;
     ++sr; qr = next_slot(qr, c.scope);
-    compute_step(c, r, sr, qr);
+    const int rq = compute_extend(c, r, sr, qr, &ak);
     if (c.spill) return kStCapacity;
-    const int rq = extend_step(c, r, sr, qr, &ak);
     if (r_max < ak) r_max = ak;
     last_forward = false;
     if (rq) return r.status;
@@ -729,18 +790,18 @@ DI int find_breakpoint(Ctx<K>& c, Dir& f, Dir& r, BP& bp) {
       if (sf + min_r - c.o >= bp.score) break;
       overlap(c, f, r, sf, qf, sr, qr, true, bp);
       ++sr; qr = next_slot(qr, c.scope);
-      compute_step(c, r, sr, qr);
+      const int rq = compute_extend(c, r, sr, qr, &ak);
       if (c.spill) return kStCapacity;
-      if (extend_step(c, r, sr, qr, &ak)) return r.status;
+      if (rq) return r.status;
     }
     const int min_f = (sf > c.scope - 1) ? sf - (c.scope - 1) : 0;
     if (min_f + sr - c.o >= bp.score) break;
     if (sf + sr > c.guard) return kStGuard;
     overlap(c, r, f, sr, qr, sf, qf, false, bp);
     ++sf; qf = next_slot(qf, c.scope);
-    compute_step(c, f, sf, qf);
+    const int fq = compute_extend(c, f, sf, qf, &ak);
     if (c.spill) return kStCapacity;
-    if (extend_step(c, f, sf, qf, &ak)) return f.status;
+    if (fq) return f.status;
     if (sr + sf >= c.max_steps) return kStMaxSteps;
     last_forward = true;
   }
@@ -897,19 +958,18 @@ DI int base_case(Ctx<K>& c, int pb, int pe, int tb, int te, int cb, int ce, int*
   team_sync<K>();
   int score = 0, ak;
   const int guard = score_guard(c, b.plen, b.tlen);
-  for (;;) {
-    if (score > guard) return kStGuard;
-    if (extend_step(c, b, score, 0, &ak)) {
-      if (b.status != kStEndReached) return kStUnattainable;
-      if (backtrace(c, b, score) != 0) return kStUnattainable;
-      *score_out = score;
-      return kStOk;
-    }
+  int done = extend_step(c, b, 0, 0, &ak);
+  while (!done) {
     ++score;
-    compute_step(c, b, score, 0);
+    done = compute_extend(c, b, score, 0, &ak);
     if (c.spill) return kStCapacity;
     if (score >= c.max_steps) return kStUnattainable;   // reported as unattainable by the base
+    if (score > guard) return kStGuard;
   }
+  if (b.status != kStEndReached) return kStUnattainable;
+  if (backtrace(c, b, score) != 0) return kStUnattainable;
+  *score_out = score;
+  return kStOk;
 }
 
 // ---------------------------------------------------------------------------
@@ -1012,6 +1072,7 @@ DI int align_pair(Ctx<K>& c, int plen, int tlen, int score_only, int* wf_score) 
 
 template <class K>
 __global__ __launch_bounds__(K::kNW > 1 ? K::kNW * kWave : kWave * kWavesPerBlock)
+__attribute__((amdgpu_waves_per_eu(4, 8)))
 void biwfa_persistent(KParams p) {
   extern __shared__ int32_t lds[];
   __shared__ int32_t tred[K::kNW > 1 ? 2 * K::kNW * kRedSlots : 1];
@@ -1080,23 +1141,23 @@ void biwfa_persistent(KParams p) {
       } else {
         c.P = gP;
         c.T = gT;
-        if constexpr (K::kNW > 1) {
-          // a team stages the pair in LDS when it fits (same padded layout)
-          const int sw = p.seq_lds_words;
-          if (pw + 3 <= sw && tw + 3 <= sw) {
-            uint64_t* sP = reinterpret_cast<uint64_t*>(lds);
-            uint64_t* sT = sP + sw;
-            __syncthreads();   // the previous pair's readers are done
-            for (int i = threadIdx.x; i < pw + 3; i += K::kNW * kWave) {
-              sP[i] = (i >= 1 && i <= pw) ? gP[i - 1] : 0ull;
-            }
-            for (int i = threadIdx.x; i < tw + 3; i += K::kNW * kWave) {
-              sT[i] = (i >= 1 && i <= tw) ? gT[i - 1] : 0ull;
-            }
-            __syncthreads();
-            c.P = sP + 1;
-            c.T = sT + 1;
+        // the pair is staged in LDS when it fits (same padded layout): one area
+        // per team, or per wave for single-wave aligners
+        const int sw = p.seq_lds_words;
+        if (pw + 3 <= sw && tw + 3 <= sw) {
+          uint64_t* sP = reinterpret_cast<uint64_t*>(lds) + (K::kNW > 1 ? 0 : 2 * sw * wv);
+          uint64_t* sT = sP + sw;
+          const int t0 = K::kNW > 1 ? (int)threadIdx.x : lane;
+          team_sync<K>();   // the previous pair's readers are done
+          for (int i = t0; i < pw + 3; i += K::kNW * kWave) {
+            sP[i] = (i >= 1 && i <= pw) ? gP[i - 1] : 0ull;
           }
+          for (int i = t0; i < tw + 3; i += K::kNW * kWave) {
+            sT[i] = (i >= 1 && i <= tw) ? gT[i - 1] : 0ull;
+          }
+          team_sync<K>();
+          c.P = sP + 1;
+          c.T = sT + 1;
         }
       }
       int wf = INT_MIN, st = kStCapacity;

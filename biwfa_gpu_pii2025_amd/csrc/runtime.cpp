@@ -191,6 +191,10 @@ struct bw_aligner {
   size_t lds_per_block_max = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t tev[kMaxTiers + 1] = {};   // tier boundaries of the last launch sequence
+  int last_ntiers = 0;
+  float tier_ms[kMaxTiers] = {};
+  bw_tuning_t tuning{};
   int32_t* d_counters = nullptr;   // per tier: {queue, count}
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
   // last single-pair result
@@ -274,6 +278,7 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
     return nullptr;
   }
   for (auto& e : a->ev) (void)hipEventCreate(&e);
+  for (auto& e : a->tev) (void)hipEventCreate(&e);
   return a;
 }
 
@@ -283,6 +288,8 @@ void bw_aligner_delete(bw_aligner_t* a) {
   if (a->stream) (void)hipStreamSynchronize(a->stream);
   dfree(a->d_counters);
   for (auto& e : a->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : a->tev)
     if (e) (void)hipEventDestroy(e);
   if (a->stream) (void)hipStreamDestroy(a->stream);
   delete a;
@@ -331,13 +338,15 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
   const int scope = q.scope;
   const int smax = base_score_bound(q);
   const int g = null_score_mask(q) + 1;
+  const bw_tuning_t& tu = a->tuning;
   // tier 0: LDS workspace, ring width 32, sequences up to 512 bases staged
   {
     Tier t;
     t.lds = true;
-    t.seq_cap = 512;
-    const int seq_words = t.seq_cap / 32 + 3;
-    t.L = make_layout(scope, 32, 0, bw::kWave, 128, 16, seq_words, false, 1);
+    t.seq_cap = tu.lds_seq_cap > 0 ? tu.lds_seq_cap : 512;
+    const int seq_words = (t.seq_cap + 31) / 32 + 3;
+    const int lds_wcap = tu.lds_ring_width > 0 ? tu.lds_ring_width : 32;
+    t.L = make_layout(scope, lds_wcap, 0, bw::kWave, 128, 16, seq_words, false, 1);
     t.waves_per_block = bw::kWavesPerBlock;
     t.dyn_lds = (size_t)t.L.total * 4 * t.waves_per_block;
     int nb = 0;
@@ -347,7 +356,7 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
       nb = std::max<int>(1, (int)((160 * 1024) / t.dyn_lds));
     if (a->attr.waves_per_cu > 0) nb = std::max(1, std::min(nb, a->attr.waves_per_cu / t.waves_per_block));
     t.blocks = a->cu_count * nb;
-    t.chunk = a->attr.queue_chunk > 0 ? a->attr.queue_chunk : 4;
+    t.chunk = tu.queue_chunk[0] > 0 ? tu.queue_chunk[0] : a->attr.queue_chunk > 0 ? a->attr.queue_chunk : 4;
     tiers.push_back(t);
   }
   // global tiers: ring width 256, 1024, ... up to the batch maximum
@@ -359,7 +368,8 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
     const int wcap = last ? wmax : w;
     Tier t;
     t.lds = false;
-    t.nw = team_waves(a, wcap);
+    const int gi = (int)tiers.size() - 1;
+    t.nw = tu.team_waves[gi] > 0 ? tu.team_waves[gi] : team_waves(a, wcap);
     const int width = std::min(2 * smax / q.e + 3, b->max_pt + 3);
     const int64_t worst = (int64_t)3 * ((smax + g) / g) * width;
     const int64_t arena = last ? worst : std::min<int64_t>(worst, (int64_t)3 * ((smax + g) / g) * std::min(wcap, width));
@@ -369,19 +379,24 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
     t.stride = t.L.total;
     const int64_t per_aligner = t.stride * 4;
     const void* kern = team_kernel(t.nw);
+    // Pairs are staged in LDS when the batch's longest sequence fits the
+    // aligner's share of the CU's LDS (the kernels run 4 waves per SIMD, i.e.
+    // 16 waves per CU).
+    const int64_t lds_cu = 160 * 1024;
     if (t.nw == 1) {
       t.waves_per_block = bw::kWavesPerBlock;
+      t.seq_lds_words = (max_seq_words * 16 <= lds_cu / 16) ? max_seq_words : 0;
+      t.dyn_lds = (size_t)t.seq_lds_words * 16 * bw::kWavesPerBlock;
       const int64_t waves = std::max<int64_t>(bw::kWavesPerBlock,
-                                              std::min<int64_t>(budget / per_aligner, (int64_t)a->cu_count * 8));
+                                              std::min<int64_t>(budget / per_aligner, (int64_t)a->cu_count * 16));
       t.blocks = (int)std::max<int64_t>(1, waves / bw::kWavesPerBlock);
       t.aligners = (int64_t)t.blocks * bw::kWavesPerBlock;
     } else {
-      // one team per workgroup; the pair is staged in LDS when the batch's
-      // longest sequence fits the team's share of the CU's LDS
+      // one team per workgroup
       t.waves_per_block = t.nw;
-      const int teams_cu = std::max(1, 20 / t.nw);
+      const int teams_cu = std::max(1, 16 / t.nw);
       const int64_t red_bytes = (int64_t)2 * t.nw * bw::kRedSlots * 4;
-      const int64_t share = (int64_t)(160 * 1024) / teams_cu - red_bytes;
+      const int64_t share = lds_cu / teams_cu - red_bytes;
       t.seq_lds_words = (max_seq_words * 16 <= share) ? max_seq_words : 0;
       t.dyn_lds = (size_t)t.seq_lds_words * 16;
       int nb = 0;
@@ -392,7 +407,7 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
       t.blocks = (int)teams;
       t.aligners = teams;
     }
-    t.chunk = 1;
+    t.chunk = tu.queue_chunk[gi + 1] > 0 ? tu.queue_chunk[gi + 1] : 1;
     tiers.push_back(t);
     if (last || (int)tiers.size() >= kMaxTiers) break;
   }
@@ -580,6 +595,7 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
   hipLaunchKernelGGL(bw::biwfa_reset, dim3(1), dim3(64), 0, s, a->d_counters, b->d_init, 2 * kMaxTiers);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(a->ev[1], s));
+  HIP_TRY(hipEventRecord(a->tev[0], s));
   for (int t = 0; t < nt; ++t) {
     const Tier& T = b->tiers[t];
     bw::KParams p = base_params(a, b);
@@ -601,9 +617,11 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
       HIP_TRY(hipLaunchKernel(team_kernel(T.nw), dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), args,
                               T.dyn_lds, s));
     }
+    HIP_TRY(hipEventRecord(a->tev[t + 1], s));
     if (t == 0) HIP_TRY(hipEventRecord(a->ev[2], s));
   }
   HIP_TRY(hipEventRecord(a->ev[3], s));
+  a->last_ntiers = nt;
   return 0;
 }
 
@@ -615,6 +633,10 @@ int bw_synchronize(bw_aligner_t* a) {
   if (hipEventElapsedTime(&total, a->ev[0], a->ev[3]) == hipSuccess) a->last_total_ms = total;
   if (hipEventElapsedTime(&main_ms, a->ev[1], a->ev[2]) == hipSuccess) a->last_main_ms = main_ms;
   if (hipEventElapsedTime(&spill_ms, a->ev[2], a->ev[3]) == hipSuccess) a->last_spill_ms = spill_ms;
+  for (int t = 0; t < a->last_ntiers; ++t) {
+    float ms = 0.f;
+    a->tier_ms[t] = hipEventElapsedTime(&ms, a->tev[t], a->tev[t + 1]) == hipSuccess ? ms : 0.f;
+  }
   return 0;
 }
 
@@ -681,6 +703,119 @@ int bw_align_batch(bw_aligner_t* a, const bw_batch_t* batch, bw_result_t* result
   if (st == 0) st = bw_dbatch_download(a, b, result);
   bw_dbatch_free(b);
   return st;
+}
+
+// ---------------- launch tuning ----------------
+
+int bw_set_tuning(bw_aligner_t* a, const bw_tuning_t* t) {
+  if (!a || !t) return BW_STATUS_INVALID_ARG;
+  const int w = t->lds_ring_width;
+  if (w != 0 && w != 16 && w != 32 && w != 64 && w != 128) {
+    set_error("lds_ring_width must be 0, 16, 32, 64 or 128");
+    return BW_STATUS_INVALID_ARG;
+  }
+  if (t->lds_seq_cap < 0 || t->lds_seq_cap > 4096) {
+    set_error("lds_seq_cap must be in [0, 4096]");
+    return BW_STATUS_INVALID_ARG;
+  }
+  for (int i = 0; i < BW_MAX_TIERS; ++i) {
+    const int v = t->team_waves[i];
+    if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8 && v != 16) {
+      set_error("team_waves must be 0, 1, 2, 4, 8 or 16");
+      return BW_STATUS_INVALID_ARG;
+    }
+    if (t->queue_chunk[i] < 0 || t->queue_chunk[i] > 1024) {
+      set_error("queue_chunk must be in [0, 1024]");
+      return BW_STATUS_INVALID_ARG;
+    }
+  }
+  a->tuning = *t;
+  return 0;
+}
+
+int bw_get_tuning(const bw_aligner_t* a, bw_tuning_t* t) {
+  if (!a || !t) return BW_STATUS_INVALID_ARG;
+  *t = a->tuning;
+  return 0;
+}
+
+int bw_last_tier_timing(const bw_aligner_t* a, float* ms, int capacity) {
+  if (!a) return BW_STATUS_INVALID_ARG;
+  for (int t = 0; t < a->last_ntiers && t < capacity; ++t) ms[t] = a->tier_ms[t];
+  return a->last_ntiers;
+}
+
+// Runs the sample under tuning `t` (upload, one warm-up, `repeats` timed runs)
+// and returns the mean per-tier and total milliseconds.
+static int time_tuning(bw_aligner_t* a, const bw_batch_t* sample, const bw_tuning_t& t, int repeats,
+                       std::vector<float>* tier_ms, float* total_ms) {
+  a->tuning = t;
+  bw_dbatch_t* b = nullptr;
+  int st = upload_impl(a, sample, &b);
+  if (st != 0) return st;
+  tier_ms->assign(kMaxTiers, 0.f);
+  *total_ms = 0.f;
+  for (int r = 0; r <= repeats && st == 0; ++r) {
+    st = bw_dbatch_align(a, b);
+    if (st == 0) st = bw_synchronize(a);
+    if (st != 0 || r == 0) continue;
+    for (int i = 0; i < a->last_ntiers; ++i) (*tier_ms)[i] += a->tier_ms[i] / repeats;
+    *total_ms += a->last_total_ms / repeats;
+  }
+  bw_dbatch_free(b);
+  return st;
+}
+
+int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, int repeats, bw_tuning_t* best) {
+  if (!a || !sample) return BW_STATUS_INVALID_ARG;
+  if (repeats < 1) repeats = 1;
+  const bw_tuning_t start = a->tuning;
+  bw_tuning_t cur = start;
+  std::vector<float> tms;
+  float total = 0.f;
+  // 1. waves per pair of every global tier: each tier's time depends only on its
+  //    own team size (which pairs reach a tier is decided by capacities)
+  static const int kTeams[] = {1, 2, 4, 8, 16};
+  float best_tier[kMaxTiers];
+  for (float& v : best_tier) v = 3.4e38f;
+  bw_tuning_t pick = cur;
+  for (int nw : kTeams) {
+    bw_tuning_t t = cur;
+    for (int i = 0; i < BW_MAX_TIERS; ++i) t.team_waves[i] = nw;
+    int st = time_tuning(a, sample, t, repeats, &tms, &total);
+    if (st != 0) { a->tuning = start; return st; }
+    for (int i = 1; i < kMaxTiers; ++i) {
+      if (tms[i] < best_tier[i]) {
+        best_tier[i] = tms[i];
+        pick.team_waves[i - 1] = nw;
+      }
+    }
+  }
+  cur = pick;
+  // 2. LDS tier: queue chunk, then ring width (whole-batch time: the ring width
+  //    moves pairs between tiers)
+  float best_total = 3.4e38f;
+  static const int kChunks[] = {1, 2, 4, 8, 16};
+  pick = cur;
+  for (int ch : kChunks) {
+    bw_tuning_t t = cur;
+    t.queue_chunk[0] = ch;
+    int st = time_tuning(a, sample, t, repeats, &tms, &total);
+    if (st != 0) { a->tuning = start; return st; }
+    if (total < best_total) { best_total = total; pick = t; }
+  }
+  cur = pick;
+  static const int kWidths[] = {16, 32, 64};
+  for (int w : kWidths) {
+    bw_tuning_t t = cur;
+    t.lds_ring_width = w;
+    int st = time_tuning(a, sample, t, repeats, &tms, &total);
+    if (st != 0) { a->tuning = start; return st; }
+    if (total < best_total) { best_total = total; pick = t; }
+  }
+  a->tuning = pick;
+  if (best) *best = pick;
+  return 0;
 }
 
 // ---------------- single pair (WFA2-style) ----------------

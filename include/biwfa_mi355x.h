@@ -193,11 +193,45 @@ int bw_last_timing(const bw_aligner_t* aligner, float* total_ms, float* main_ms,
 
 /* ---------------- (iv) scheduling ---------------- */
 
-/* Selects the pair order of later bw_dbatch_align calls (bw_schedule_t).  For
- * BW_SCHEDULE_LEARNED, `predicted_cost` holds one value per pair (host); larger
- * values start first.  The order never changes any pair's result. */
+/* Selects the pair order and starting tier of later bw_dbatch_align calls
+ * (bw_schedule_t).  For BW_SCHEDULE_LEARNED, `predicted_score` holds one
+ * predicted WFA score (penalty units, >= 0) per pair (host): pairs start in the
+ * first tier whose ring is wide enough for it and are ordered by predicted work,
+ * largest first.  The schedule never changes any pair's result. */
 int bw_set_schedule(bw_aligner_t* aligner, bw_dbatch_t* dbatch, int schedule,
-                    const float* predicted_cost);
+                    const float* predicted_score);
+
+/* ---------------- launch tuning ---------------- */
+
+/* No counterpart in the reference (its kernel runs one pair per 32-thread block).
+ * A batch runs through tiers of growing wavefront-ring width: tier 0 keeps the
+ * whole workspace in LDS with one wave per pair; global tier i (i = 0, 1, ...)
+ * has a ring 256 * 4^i diagonals wide in HBM (the last one as wide as the batch
+ * needs) and aligns each pair with a team of team_waves[i] waves (64 lanes each)
+ * of one workgroup.  A pair that outgrows a tier moves to the next one on the
+ * device.  Zero fields take the built-in rule.  Tuning changes speed only. */
+#define BW_MAX_TIERS 8
+typedef struct {
+  int lds_ring_width;                   /* tier 0 ring width: 16, 32, 64 or 128 (0: 32)      */
+  int lds_seq_cap;                      /* tier 0 longest staged sequence, <= 4096 (0: 512) */
+  int team_waves[BW_MAX_TIERS];         /* global tier i: 1, 2, 4, 8 or 16 waves per pair    */
+  int queue_chunk[BW_MAX_TIERS];        /* tier i (0 = LDS tier): pairs per queue grab       */
+} bw_tuning_t;
+
+/* Tuning used by later uploads (bw_dbatch_upload / bw_align_batch plan their tiers
+ * with it).  bw_get_tuning returns the current one (zeros = built-in rules). */
+int bw_set_tuning(bw_aligner_t* aligner, const bw_tuning_t* tuning);
+int bw_get_tuning(const bw_aligner_t* aligner, bw_tuning_t* tuning);
+
+/* Per-tier milliseconds of the last bw_dbatch_align (valid after bw_synchronize):
+ * ms[0] the LDS tier, ms[1 + i] global tier i.  Returns the number of tiers. */
+int bw_last_tier_timing(const bw_aligner_t* aligner, float* ms, int capacity);
+
+/* Times the sample batch under candidate tunings and keeps the fastest: team
+ * waves per global tier (each tier timed on its own), then the LDS tier's queue
+ * chunk and ring width (whole batch).  `repeats` timed runs per candidate (after
+ * one warm-up).  The result is set on the aligner and copied to *best. */
+int bw_autotune(bw_aligner_t* aligner, const bw_batch_t* sample, int repeats, bw_tuning_t* best);
 
 /* ---------------- (v) learned cost model ---------------- */
 
