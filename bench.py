@@ -42,6 +42,21 @@ def algorithmic_bytes(p_len, t_len, nruns):
     return int(np.sum(8 * words) + len(p_len) * (24 + 4 + 16) + 4 * int(np.sum(nruns)))
 
 
+def traffic_bytes(workload, n):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's readings), when they
+    were taken on this same workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "c2_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or int(t.get("pairs", -1)) != n:
+        return None
+    return int(t["traffic_bytes"])
+
+
 def _per_pair_mismatches(off_a, runs_a, off_b, runs_b, m):
     """Number of pairs among the first m whose run lists differ."""
     if np.array_equal(off_a[:m + 1], off_b[:m + 1]) and \
@@ -114,7 +129,23 @@ def main():
     ap.add_argument("--autotune", type=int, default=0, metavar="N",
                     help="tune launch shapes on an N-pair sample of the workload before timing")
     ap.add_argument("--tuning", default=None, help="JSON launch tuning (bw_tuning_t fields)")
+    ap.add_argument("--compare-schedules", action="store_true",
+                    help="train the cost model, then time the plain length-sorted and the learned "
+                         "schedule on the same batch (value = learned)")
+    ap.add_argument("--train-pairs", type=int, default=None,
+                    help="training pairs for the cost model (default: the full 50,000-pair set)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.empty(args.calibrate_copy << 18, dtype=torch.int32, device="cuda").fill_(1)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied %d MiB three times" % args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
               "(pairs would shard across GPUs with no exchange but a final gather); "
@@ -139,22 +170,59 @@ def main():
         tuned = aligner.autotune(sample, repeats=2)
         log("autotuned on %d pairs in %.1f s: %s" % (len(sample), time.time() - t0, json.dumps(tuned)))
     db = aligner.upload(packed)
-    for w in range(args.warmup):
-        db.align()
-        log("warmup %d done (%.2f ms)" % (w, db.timing()[0]))
-    kern_main, kern_spill, kern_total, kern_tiers = [], [], [], []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        db.align()
-        tot, main_ms, spill_ms = db.timing()
-        kern_total.append(tot); kern_main.append(main_ms); kern_spill.append(spill_ms)
-        kern_tiers.append(aligner.tier_timing())
-    t_end = time.perf_counter()
-    ms_per_step = (t_end - t_start) * 1000.0 / args.steps
-    tier_ms = [round(float(v), 4) for v in np.mean(np.array(kern_tiers), axis=0)]
-    log("timed %d steps: %.3f ms/step (kernels %.3f ms: main %.3f + spill %.3f; tiers %s)" % (
-        args.steps, ms_per_step, np.mean(kern_total), np.mean(kern_main), np.mean(kern_spill), tier_ms))
+
+    def timed(label):
+        for w in range(args.warmup):
+            db.align()
+            log("%swarmup %d done (%.2f ms)" % (label, w, db.timing()[0]))
+        km, ks, kt, ktiers = [], [], [], []
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            db.align()
+            tot, main_ms, spill_ms = db.timing()
+            kt.append(tot); km.append(main_ms); ks.append(spill_ms)
+            ktiers.append(aligner.tier_timing())
+        t_end = time.perf_counter()
+        ms = (t_end - t_start) * 1000.0 / args.steps
+        tiers = [round(float(v), 4) for v in np.mean(np.array(ktiers), axis=0)]
+        log("%stimed %d steps: %.3f ms/step (kernels %.3f ms: main %.3f + spill %.3f; tiers %s; "
+            "tier pairs %s)" % (label, args.steps, ms, np.mean(kt), np.mean(km), np.mean(ks), tiers,
+                                 aligner.tier_pairs()))
+        return ms, km, ks, kt, tiers
+
+    sched_info = None
+    if args.compare_schedules:
+        from biwfa_gpu_pii2025_amd import costmodel as cm
+        plain_ms = timed("plain: ")[0]
+        plain_res = db.download()
+        t0 = time.time()
+        model, tinfo = cm.train_cost_model(num_pairs=args.train_pairs, epochs=30, seed=0)
+        train_s = time.time() - t0
+        log("cost model trained on %d aligner-labelled pairs in %.1f s (mse %.5f -> %.5f)" % (
+            tinfo["pairs"], train_s, tinfo["initial_mse"], tinfo["final_mse"]))
+        t0 = time.time()
+        pred = cm.learned_schedule(db, model, packed)
+        sched_s = time.time() - t0
+        true_score = plain_res.wf_score.astype(np.float64)
+        err = np.abs(np.log2(1 + pred) - np.log2(1 + np.maximum(true_score, 0)))
+        rank_p = np.argsort(np.argsort(pred))
+        rank_t = np.argsort(np.argsort(true_score))
+        spearman = float(np.corrcoef(rank_p, rank_t)[0, 1])
+        sched_info = {"plain_ms_per_step": round(plain_ms, 4), "plain_pairs_s": round(n / plain_ms * 1e3, 1),
+                      "train_pairs": tinfo["pairs"], "train_s": round(train_s, 2),
+                      "train_mse": [round(tinfo["initial_mse"], 6), round(tinfo["final_mse"], 6)],
+                      "predict_and_schedule_s": round(sched_s, 3),
+                      "score_prediction": {"median_abs_log2_err": round(float(np.median(err)), 4),
+                                           "spearman": round(spearman, 4)}}
+        model.close()
+    ms_per_step, kern_main, kern_spill, kern_total, tier_ms = timed("learned: " if args.compare_schedules else "")
     res = db.download()
+    if sched_info is not None:
+        same = (np.array_equal(res.score, plain_res.score) and np.array_equal(res.cigar_offset, plain_res.cigar_offset)
+                and np.array_equal(res.cigar_runs, plain_res.cigar_runs))
+        sched_info["learned_pairs_s"] = round(n / ms_per_step * 1e3, 1)
+        sched_info["learned_vs_plain"] = round(sched_info["plain_ms_per_step"] / ms_per_step, 3)
+        sched_info["identical_results"] = bool(same)
     bad_status = int(np.sum(res.status != 0))
     nruns = np.diff(res.cigar_offset)
 
@@ -177,7 +245,9 @@ def main():
         "data": "synthetic (seeded generator, seed %d)" % cfg.seed,
         "config": {
             "workload": "%s: %s; %d pairs; gap-affine x=4 o=6 e=2; score+CIGAR; %s schedule" % (
-                cfg.name, cfg.description, n, "length-sorted" if args.schedule == "length" else "input-order"),
+                cfg.name, cfg.description, n,
+                "learned (cost model)" if args.compare_schedules else
+                "length-sorted" if args.schedule == "length" else "input-order"),
             "pairs": n,
             "pattern_length": [int(packed.p_len.min()), int(packed.p_len.max())],
             "penalties": list(bw.configs.PENALTIES),
@@ -186,6 +256,7 @@ def main():
         "kernel_ms": {"total": round(float(np.mean(kern_total)), 4), "main": round(float(np.mean(kern_main)), 4),
                       "spill": round(float(np.mean(kern_spill)), 4), "tiers": tier_ms},
         "tuning": aligner.tuning,
+        "schedules": sched_info,
         "status_errors": bad_status,
         "roofline": {
             "bound": "hbm",
@@ -193,8 +264,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
-            "kernel": "biwfa_persistent<true> (LDS workspace)",
+            "traffic": traffic_bytes(args.workload, n),
+            "kernel": "biwfa_persistent<Cfg<true,1>> (LDS tier, one wave per pair)",
             "algorithmic_bytes_per_launch": alg_bytes,
         },
     }

@@ -123,6 +123,7 @@ def lib():
         L.bw_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Tuning)]
         L.bw_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Tuning)]
         L.bw_last_tier_timing.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.bw_last_tier_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.bw_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch), ctypes.c_int, ctypes.POINTER(_Tuning)]
         L.bw_dataset_lengths.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_generate.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p,
@@ -463,6 +464,12 @@ class WavefrontAligner:
         ms = np.zeros(MAX_TIERS, np.float32)
         n = self._L.bw_last_tier_timing(self._h, _ptr(ms), MAX_TIERS)
         return ms[:max(0, n)].tolist()
+
+    def tier_pairs(self):
+        """Pairs each tier took on in the last device-batch alignment (re-runs included)."""
+        cnt = np.zeros(MAX_TIERS, np.int32)
+        n = self._L.bw_last_tier_pairs(self._h, _ptr(cnt), MAX_TIERS)
+        return cnt[:max(0, n)].tolist()
 
 
 class DeviceBatch:

@@ -194,6 +194,7 @@ struct bw_aligner {
   hipEvent_t tev[kMaxTiers + 1] = {};   // tier boundaries of the last launch sequence
   int last_ntiers = 0;
   float tier_ms[kMaxTiers] = {};
+  int32_t tier_pairs[kMaxTiers] = {};   // pairs each tier took on (initial + spilled in)
   bw_tuning_t tuning{};
   int32_t* d_counters = nullptr;   // per tier: {queue, count}
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
@@ -637,6 +638,9 @@ int bw_synchronize(bw_aligner_t* a) {
     float ms = 0.f;
     a->tier_ms[t] = hipEventElapsedTime(&ms, a->tev[t], a->tev[t + 1]) == hipSuccess ? ms : 0.f;
   }
+  int32_t cnt[2 * kMaxTiers];
+  HIP_TRY(hipMemcpy(cnt, a->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  for (int t = 0; t < kMaxTiers; ++t) a->tier_pairs[t] = t < a->last_ntiers ? cnt[2 * t + 1] : 0;
   return 0;
 }
 
@@ -742,6 +746,12 @@ int bw_get_tuning(const bw_aligner_t* a, bw_tuning_t* t) {
 int bw_last_tier_timing(const bw_aligner_t* a, float* ms, int capacity) {
   if (!a) return BW_STATUS_INVALID_ARG;
   for (int t = 0; t < a->last_ntiers && t < capacity; ++t) ms[t] = a->tier_ms[t];
+  return a->last_ntiers;
+}
+
+int bw_last_tier_pairs(const bw_aligner_t* a, int32_t* pairs, int capacity) {
+  if (!a) return BW_STATUS_INVALID_ARG;
+  for (int t = 0; t < a->last_ntiers && t < capacity; ++t) pairs[t] = a->tier_pairs[t];
   return a->last_ntiers;
 }
 

@@ -227,6 +227,11 @@ int bw_get_tuning(const bw_aligner_t* aligner, bw_tuning_t* tuning);
  * ms[0] the LDS tier, ms[1 + i] global tier i.  Returns the number of tiers. */
 int bw_last_tier_timing(const bw_aligner_t* aligner, float* ms, int capacity);
 
+/* Pairs each tier took on in the last bw_dbatch_align (its initial share plus the
+ * pairs the previous tier passed on); the sum minus the batch size counts re-runs.
+ * Valid after bw_synchronize.  Returns the number of tiers. */
+int bw_last_tier_pairs(const bw_aligner_t* aligner, int32_t* pairs, int capacity);
+
 /* Times the sample batch under candidate tunings and keeps the fastest: team
  * waves per global tier (each tier timed on its own), then the LDS tier's queue
  * chunk and ring width (whole batch).  `repeats` timed runs per candidate (after
