@@ -26,6 +26,9 @@ constexpr int kWave = 64;              // gfx950 wavefront size
 constexpr int kWavesPerBlock = 4;      // independent single-wave aligners per workgroup
 constexpr int kMaxTeam = 16;           // waves cooperating on one pair (one workgroup)
 constexpr int kRedSlots = 3 * kMaxScope;   // team reduction values per wave
+constexpr int kSubInts = 10;           // recursion stack entry (a sub-problem)
+constexpr int kItemInts = 12;          // shared recursion task: ready flag + sub-problem
+constexpr int kQctlInts = 6;           // per tier: {queue, count, head, tail, busy, error}
 
 // Kernel configuration: kLds keeps the whole workspace and both sequences in
 // LDS (one wave per pair); otherwise the workspace is a global slab shared by a
@@ -93,6 +96,18 @@ struct KParams {
   int32_t* gws;                  // global workspace (global kernels)
   int64_t gws_stride;            // ints per aligner (wave or team)
   int32_t seq_lds_words;         // team kernels: LDS words per sequence for staging (0: none)
+  // Global tiers with score + CIGAR run the BiWFA recursion as shared tasks:
+  // a sub-problem split off at a breakpoint may be taken by any aligner of the
+  // tier.  Leaves write their runs at position pb+tb of the pair's run buffer;
+  // the aligner that finishes a pair's last sub-problem merges them.
+  int32_t task;                  // 1: shared-task recursion
+  int32_t* items;                // item_cap x kItemInts
+  int32_t item_cap;
+  int32_t epoch;                 // ready-flag value of this launch (never 0)
+  int32_t* qctl;                 // this tier's {queue, count, head, tail, busy, error}
+  int32_t* pend;                 // per pair: unfinished sub-problems
+  unsigned long long* pfail;     // per pair: leftmost failure, (pb+tb) << 16 | -status
+  int32_t* pflag;                // per pair: 1 workspace capacity exceeded, 2 run overflow
   WsLayout L;
 };
 

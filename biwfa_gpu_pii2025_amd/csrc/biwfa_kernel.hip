@@ -452,7 +452,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   // far as the compiler knows, so without the grouping every chunk would wait
   // out a full memory latency on its own.  (The LDS ring is fast enough and
   // narrow enough for one chunk at a time.)
-  constexpr int kG = K::kLds ? 1 : 4;
+  constexpr int kG = K::kLds ? 1 : 3;
   constexpr int kStride = K::kNW * kWave;
   for (int kb0 = lo + c.wid * kWave; kb0 <= hi; kb0 += kG * kStride) {
     int ins1[kG], del1[kG], misms[kG];
@@ -976,45 +976,98 @@ DI int base_case(Ctx<K>& c, int pb, int pe, int tb, int te, int cb, int ce, int*
 // recursion driver: wavefront_bialign_alignment (wavefront_bialign.c:581-658)
 
 struct Sub {
-  int pb, pe, tb, te, cb, ce, rem, lvl;
+  int pair, pb, pe, tb, te, cb, ce, rem, lvl;
 };
 template <class K>
 DI void push(Ctx<K>& c, int& sp, const Sub& s) {
   if (sp >= c.L.scap) { c.spill = 1; return; }
-  int* e = c.pv + c.L.stack + sp * 8;
-  e[0] = s.pb; e[1] = s.pe; e[2] = s.tb; e[3] = s.te;
-  e[4] = s.cb; e[5] = s.ce; e[6] = s.rem; e[7] = s.lvl;
+  int* e = c.pv + c.L.stack + sp * kSubInts;
+  e[0] = s.pair; e[1] = s.pb; e[2] = s.pe; e[3] = s.tb; e[4] = s.te;
+  e[5] = s.cb; e[6] = s.ce; e[7] = s.rem; e[8] = s.lvl;
   ++sp;
 }
 template <class K>
 DI Sub pop(Ctx<K>& c, int& sp) {
   --sp;
-  const int* e = c.pv + c.L.stack + sp * 8;
-  return Sub{uni(e[0]), uni(e[1]), uni(e[2]), uni(e[3]), uni(e[4]), uni(e[5]), uni(e[6]), uni(e[7])};
+  const int* e = c.pv + c.L.stack + sp * kSubInts;
+  return Sub{uni(e[0]), uni(e[1]), uni(e[2]), uni(e[3]), uni(e[4]), uni(e[5]), uni(e[6]), uni(e[7]),
+             uni(e[8])};
 }
 
-// Aligns the staged pair.  Returns the final WFA status (wavefront_bialign.c:722-729).
+constexpr int kSubLeaf = 1;    // output emitted
+constexpr int kSubSplit = 2;   // breakpoint found: two children
+
+// One sub-problem of wavefront_bialign_alignment (wavefront_bialign.c:581-658):
+// trivial cases, the base case below the score threshold, or the breakpoint
+// search with its exception path (:520-547).  Returns kSubLeaf, kSubSplit (the
+// children in *left / *right) or a failure status.  *wf_score receives the
+// level-0 score.
 template <class K>
-DI int align_pair(Ctx<K>& c, int plen, int tlen, int score_only, int* wf_score) {
-  *wf_score = INT_MIN;
-  if (score_only) {
-    // wavefront_bialign_compute_score (wavefront_bialign.c:662-699)
-    Dir f{0, plen, 0, tlen, 0, 0, CM, CM, 0, kStOk, 0};
-    Dir r{0, plen, 0, tlen, 1, 1, CM, CM, 0, kStOk, 0};
-    BP bp;
-    const int st = find_breakpoint(c, f, r, bp);
+DI int run_sub(Ctx<K>& c, const Sub& s, Sub* left, Sub* right, int* wf_score) {
+  const int pl = s.pe - s.pb, tl = s.te - s.tb;
+  if (tl == 0) { out_append(c, kOpD, pl); return kSubLeaf; }
+  if (pl == 0) { out_append(c, kOpI, tl); return kSubLeaf; }
+  int bscore = 0, st;
+  if (s.rem <= kFallbackMinScore) {
+    st = base_case(c, s.pb, s.pe, s.tb, s.te, s.cb, s.ce, &bscore);
     if (c.spill) return kStCapacity;
-    if (st != kStOk && st != kStEndReached)
-      return (st == kStMaxSteps || st == kStGuard) ? st : kStUnattainable;
-    int s = bp.score;
-    if (st == kStEndReached) s = (f.status == kStEndReached) ? f.status_score : r.status_score;
-    *wf_score = s;
-    c.alg_score = (c.um == 0) ? -s : ((-c.um) * (plen + tlen) - s) / 2;
-    return 0;
+    if (s.lvl == 0) *wf_score = bscore;
+    return st == kStOk ? kSubLeaf : st;
   }
+  Dir f{s.pb, pl, s.tb, tl, 0, 0, s.cb, s.ce, 0, kStOk, 0};
+  Dir r{s.pb, pl, s.tb, tl, 1, 1, s.ce, s.cb, 0, kStOk, 0};
+  BP bp;
+  st = find_breakpoint(c, f, r, bp);
+  if (c.spill) return kStCapacity;
+  if (st != kStOk) {   // wavefront_bialign_find_breakpoint_exception (:520-547)
+    if (st != kStEndReached) return st;
+    const int reached = (f.status == kStEndReached) ? f.status_score : r.status_score;
+    if (reached > kRecoveryMinScore) return kStEndUnreachable;
+    st = base_case(c, s.pb, s.pe, s.tb, s.te, s.cb, s.ce, &bscore);
+    if (c.spill) return kStCapacity;
+    if (s.lvl == 0) *wf_score = bscore;
+    return st == kStOk ? kSubLeaf : st;
+  }
+  if (s.lvl == 0) *wf_score = bp.score;
+  const int bh = bp.of, bv = bp.of - bp.kf;
+  *right = Sub{s.pair, s.pb + bv, s.pe, s.tb + bh, s.te, bp.comp, s.ce, bp.sr, s.lvl + 1};
+  *left = Sub{s.pair, s.pb, s.pb + bv, s.tb, s.tb + bh, s.cb, bp.comp, bp.sf, s.lvl + 1};
+  return kSubSplit;
+}
+
+// Final status of a failed alignment (wavefront_bialign.c:722-729).
+DI int final_status(int st) {
+  if (st == kStMaxSteps || st == kStGuard || st == kStCapacity) return st;
+  return kStUnattainable;
+}
+
+// wavefront_bialign_compute_score (wavefront_bialign.c:662-699)
+template <class K>
+DI int score_pair(Ctx<K>& c, int plen, int tlen, int* wf_score) {
+  *wf_score = INT_MIN;
+  Dir f{0, plen, 0, tlen, 0, 0, CM, CM, 0, kStOk, 0};
+  Dir r{0, plen, 0, tlen, 1, 1, CM, CM, 0, kStOk, 0};
+  BP bp;
+  const int st = find_breakpoint(c, f, r, bp);
+  if (c.spill) return kStCapacity;
+  if (st != kStOk && st != kStEndReached)
+    return (st == kStMaxSteps || st == kStGuard) ? st : kStUnattainable;
+  int s = bp.score;
+  if (st == kStEndReached) s = (f.status == kStEndReached) ? f.status_score : r.status_score;
+  *wf_score = s;
+  c.alg_score = (c.um == 0) ? -s : ((-c.um) * (plen + tlen) - s) / 2;
+  return 0;
+}
+
+// Aligns the staged pair depth first on this aligner alone, emitting the runs
+// in order.  Returns the final WFA status (wavefront_bialign.c:722-729).
+template <class K>
+DI int align_pair(Ctx<K>& c, int pair, int plen, int tlen, int score_only, int* wf_score) {
+  if (score_only) return score_pair(c, plen, tlen, wf_score);
+  *wf_score = INT_MIN;
   int sp = 0;
   const bool min_length = imax(plen, tlen) <= kFallbackMinLength;
-  push(c, sp, Sub{0, plen, 0, tlen, CM, CM, min_length ? 0 : INT_MAX, 0});
+  push(c, sp, Sub{pair, 0, plen, 0, tlen, CM, CM, min_length ? 0 : INT_MAX, 0});
   wave_sync<K::kLds>();
   int status = kStOk;
   int iters = 0;
@@ -1022,49 +1075,344 @@ DI int align_pair(Ctx<K>& c, int plen, int tlen, int score_only, int* wf_score) 
   while (sp > 0) {
     if (++iters > max_iters) { status = kStGuard; break; }
     const Sub s = pop(c, sp);
-    const int pl = s.pe - s.pb, tl = s.te - s.tb;
-    if (tl == 0) { out_append(c, kOpD, pl); continue; }
-    if (pl == 0) { out_append(c, kOpI, tl); continue; }
-    int bscore = 0, st;
-    if (s.rem <= kFallbackMinScore) {
-      st = base_case(c, s.pb, s.pe, s.tb, s.te, s.cb, s.ce, &bscore);
-      if (c.spill) return kStCapacity;
-      if (s.lvl == 0) *wf_score = bscore;
-      if (st != kStOk) { status = st; break; }
-      continue;
-    }
-    Dir f{s.pb, pl, s.tb, tl, 0, 0, s.cb, s.ce, 0, kStOk, 0};
-    Dir r{s.pb, pl, s.tb, tl, 1, 1, s.ce, s.cb, 0, kStOk, 0};
-    BP bp;
-    st = find_breakpoint(c, f, r, bp);
+    Sub l, r;
+    const int res = run_sub(c, s, &l, &r, wf_score);
     if (c.spill) return kStCapacity;
-    if (st != kStOk) {   // wavefront_bialign_find_breakpoint_exception (:520-547)
-      if (st == kStEndReached) {
-        const int reached = (f.status == kStEndReached) ? f.status_score : r.status_score;
-        if (reached <= kRecoveryMinScore) {
-          st = base_case(c, s.pb, s.pe, s.tb, s.te, s.cb, s.ce, &bscore);
-          if (c.spill) return kStCapacity;
-          if (s.lvl == 0) *wf_score = bscore;
-          if (st != kStOk) { status = st; break; }
-          continue;
-        }
-        status = kStEndUnreachable;
-        break;
-      }
-      status = st;
-      break;
-    }
-    if (s.lvl == 0) *wf_score = bp.score;
-    const int bh = bp.of, bv = bp.of - bp.kf;
-    push(c, sp, Sub{s.pb + bv, s.pe, s.tb + bh, s.te, bp.comp, s.ce, bp.sr, s.lvl + 1});
-    push(c, sp, Sub{s.pb, s.pb + bv, s.tb, s.tb + bh, s.cb, bp.comp, bp.sf, s.lvl + 1});
+    if (res == kSubLeaf) continue;
+    if (res != kSubSplit) { status = res; break; }
+    push(c, sp, r);
+    push(c, sp, l);
     if (c.spill) return kStCapacity;
     wave_sync<K::kLds>();
   }
   out_flush(c);
   if (status == kStOk) return 0;
-  if (status == kStMaxSteps || status == kStGuard) return status;
-  return kStUnattainable;
+  return final_status(status);
+}
+
+// ---------------------------------------------------------------------------
+// shared recursion tasks (global tiers, score + CIGAR)
+//
+// A pair is taken from the tier's list by one aligner, which runs its root
+// sub-problem.  At every breakpoint the right child is published to the tier's
+// item queue (when it has room; else kept) and the left child stays on the
+// aligner's own stack, so idle aligners take over halves of long pairs.
+//   * Output: a leaf covering [pb,pe) x [tb,te) has at most (pe-pb)+(te-tb)
+//     runs and writes them at pb+tb of the pair's run buffer (zeroed when the
+//     pair starts); leaves tile the alignment path in order, so their slots
+//     never overlap and the zero gaps can be squeezed out afterwards.
+//   * pend[pair] counts unfinished sub-problems (+2 per split before the
+//     children exist, -1 per finished one); whoever takes it to zero merges the
+//     runs (maximal runs, as the sequential emitter produces them) and writes
+//     the pair's results.
+//   * Failures: the leftmost failing sub-problem is the one depth-first order
+//     meets first (its left neighbours are finished leaves, its ancestors
+//     succeeded), so its status is the pair's status.
+//   * Publishing: tail claimed with an atomic, fields written, agent-scope
+//     fence, then the ready flag set to the launch's epoch; a taker spins on
+//     the flag and fences before reading.
+
+constexpr int kFlagSpill = 1, kFlagRuns = 2;
+// an aligner that sees no progress for this long gives up (a lost item can
+// only be a defect); 100 MHz constant clock
+constexpr uint64_t kStallTicks = 20ull * 100000000ull;
+
+DI int atomic_ld(const int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// the leader's values a, b to the whole team
+template <class K>
+DI void team_bcast2(Ctx<K>& c, int& a, int& b) {
+  if constexpr (K::kNW == 1) {
+    a = uni(__shfl(a, 0, kWave));
+    b = uni(__shfl(b, 0, kWave));
+  } else {
+    int32_t* buf = team_buf(c);
+    if (c.leader) { buf[0] = a; buf[1] = b; }
+    __syncthreads();
+    a = uni(buf[0]);
+    b = uni(buf[1]);
+  }
+}
+
+// Stages the pair's sequences in the aligner's LDS area when they fit (same
+// padded layout as the LDS tier), else points at HBM.
+template <class K>
+DI void stage_pair(Ctx<K>& c, const KParams& p, int pair, int32_t* lds, int wv) {
+  const int plen = uni(p.p_len[pair]), tlen = uni(p.t_len[pair]);
+  const uint64_t* gP = p.words + p.p_woff[pair];
+  const uint64_t* gT = p.words + p.t_woff[pair];
+  c.P = gP;
+  c.T = gT;
+  const int pw = (plen + 31) >> 5, tw = (tlen + 31) >> 5;
+  const int sw = p.seq_lds_words;
+  if (pw + 3 <= sw && tw + 3 <= sw) {
+    uint64_t* sP = reinterpret_cast<uint64_t*>(lds) + (K::kNW > 1 ? 0 : 2 * sw * wv);
+    uint64_t* sT = sP + sw;
+    const int t0 = K::kNW > 1 ? (int)threadIdx.x : c.lane;
+    team_sync<K>();   // the previous pair's readers are done
+    for (int i = t0; i < pw + 3; i += K::kNW * kWave) sP[i] = (i >= 1 && i <= pw) ? gP[i - 1] : 0ull;
+    for (int i = t0; i < tw + 3; i += K::kNW * kWave) sT[i] = (i >= 1 && i <= tw) ? gT[i - 1] : 0ull;
+    team_sync<K>();
+    c.P = sP + 1;
+    c.T = sT + 1;
+  }
+}
+
+// Publishes sub-problem s to the tier's queue; returns 0 when the queue is full.
+template <class K>
+DI int publish(Ctx<K>& c, const KParams& p, const Sub& s) {
+  int ok = 0, unused = 0;
+  if (c.leader) {
+    const int slot = atomicAdd(&p.qctl[3], 1);
+    if (slot < p.item_cap) {
+      int32_t* it = p.items + (int64_t)slot * kItemInts;
+      it[1] = s.pair; it[2] = s.pb; it[3] = s.pe; it[4] = s.tb; it[5] = s.te;
+      it[6] = s.cb; it[7] = s.ce; it[8] = s.rem; it[9] = s.lvl;
+      __threadfence();
+      __hip_atomic_store(it, p.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ok = 1;
+    }
+  }
+  team_bcast2(c, ok, unused);
+  return ok;
+}
+
+// Merges the pair's leaf runs in place (zero slots dropped, equal neighbours
+// joined) and writes its results.  Wave 0 alone reads and rewrites the buffer.
+template <class K>
+DI void finish_pair(Ctx<K>& c, const KParams& p, int pair) {
+  const int plen = uni(p.p_len[pair]), tlen = uni(p.t_len[pair]);
+  const int flags = uni(atomic_ld(&p.pflag[pair]));
+  const unsigned long long fail = __hip_atomic_load(&p.pfail[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int fail_lo = uni((int)(uint32_t)(fail & 0xffffffffull)), fail_hi = uni((int)(uint32_t)(fail >> 32));
+  const bool failed = !(fail_lo == -1 && fail_hi == -1);
+  if (flags & kFlagSpill) {
+    if (p.next_list) {      // re-run by the next tier with larger capacities
+      if (c.leader) {
+        const int slot = atomicAdd(p.next_count, 1);
+        p.next_list[slot] = pair;
+      }
+      return;
+    }
+  }
+  int st = 0;
+  if (flags & kFlagSpill) st = kStCapacity;
+  else if (failed) st = final_status(-(fail_lo & 0xffff));
+  else if (flags & kFlagRuns) st = kStCapacity;
+  uint32_t* base = p.runs + p.runs_off[pair];
+  c.out = base;
+  c.out_cap = plen + tlen + 1;
+  c.out_n = 0; c.pend_op = -1; c.pend_len = 0; c.alg_score = 0; c.cap_err = 0;
+  if (st == 0 && c.wid == 0) {
+    const int total = plen + tlen;
+    for (int i0 = 0; i0 < total; i0 += kWave) {
+      const int i = i0 + c.lane;
+      const uint32_t v = i < total ? base[i] : 0u;
+      uint64_t b = __ballot(v != 0u);
+      while (b) {
+        const int j = __builtin_ctzll(b);
+        b &= b - 1;
+        const uint32_t run = (uint32_t)rdl((int)v, j);
+        out_append(c, (int)(run & 15u), (int)(run >> 4));
+      }
+    }
+    out_flush(c);
+  }
+  if (c.leader) {
+    p.status[pair] = st;
+    p.score[pair] = (st == 0 && c.out_n > 0) ? c.alg_score : -1;
+    p.nruns[pair] = st == 0 ? c.out_n : 0;
+  }
+  team_sync<K>();
+}
+
+// One finished sub-problem of `pair`; the last one finishes the pair.
+template <class K>
+DI void sub_done(Ctx<K>& c, const KParams& p, int pair) {
+  int old = 0, unused = 0;
+  if (c.leader) {
+    __threadfence();
+    old = atomicSub(&p.pend[pair], 1);
+  }
+  team_bcast2(c, old, unused);
+  if (old == 1) {
+    __threadfence();
+    finish_pair(c, p, pair);
+  }
+}
+
+// Runs `first` and every sub-problem it leaves on this aligner's stack.
+template <class K>
+DI void run_task(Ctx<K>& c, const KParams& p, const Sub& first, int& staged, int32_t* lds, int wv) {
+  int sp = 0;
+  push(c, sp, first);
+  wave_sync<K::kLds>();
+  int iters = 0;
+  const int max_iters = 8 * ((first.pe - first.pb) + (first.te - first.tb)) + 64;
+  while (sp > 0) {
+    const Sub s = pop(c, sp);
+    if (s.pair != staged) {
+      stage_pair(c, p, s.pair, lds, wv);
+      staged = s.pair;
+    }
+    c.out = p.runs + p.runs_off[s.pair] + s.pb + s.tb;
+    c.out_cap = (s.pe - s.pb) + (s.te - s.tb);
+    c.out_n = 0; c.pend_op = -1; c.pend_len = 0; c.alg_score = 0;
+    c.spill = 0; c.cap_err = 0; c.arena_top = 0;
+    int wf = INT_MIN;
+    Sub l, r;
+    int res = (++iters > max_iters) ? kStGuard : run_sub(c, s, &l, &r, &wf);
+    if (c.spill) res = kStCapacity;
+    if (res == kSubLeaf) {
+      out_flush(c);
+      if (c.cap_err && c.leader) atomicOr(&p.pflag[s.pair], kFlagRuns);
+    } else if (res == kSubSplit) {
+      if (c.leader) atomicAdd(&p.pend[s.pair], 2);
+      int lost = 0;   // children the full stack could not take (finished as failed)
+      if (!publish(c, p, r)) {
+        push(c, sp, r);
+        if (c.spill) { ++lost; c.spill = 0; }
+      }
+      push(c, sp, l);
+      if (c.spill) { ++lost; c.spill = 0; }
+      wave_sync<K::kLds>();
+      if (lost) {
+        if (c.leader) atomicOr(&p.pflag[s.pair], kFlagSpill);
+        for (int q = 0; q < lost; ++q) sub_done(c, p, s.pair);
+      }
+    } else if (c.leader) {
+      if (res == kStCapacity) {
+        atomicOr(&p.pflag[s.pair], kFlagSpill);
+      } else {
+        const unsigned long long key = ((unsigned long long)(uint32_t)(s.pb + s.tb) << 16) |
+                                       (unsigned long long)((uint32_t)(-res) & 0xffffu);
+        atomicMin(&p.pfail[s.pair], key);
+      }
+    }
+    if (s.lvl == 0 && wf != INT_MIN && c.leader) p.wf_score[s.pair] = wf;
+    sub_done(c, p, s.pair);
+  }
+}
+
+// Sets a pair up for shared-task alignment and runs its root (score-only: the
+// whole alignment, wavefront_bialign_compute_score).
+template <class K>
+DI void start_pair(Ctx<K>& c, const KParams& p, int pair, int& staged, int32_t* lds, int wv) {
+  const int plen = uni(p.p_len[pair]), tlen = uni(p.t_len[pair]);
+  if (p.score_only) {
+    if (pair != staged) {
+      stage_pair(c, p, pair, lds, wv);
+      staged = pair;
+    }
+    c.spill = 0; c.arena_top = 0;
+    int wf = INT_MIN;
+    const int st = score_pair(c, plen, tlen, &wf);
+    if (c.spill && p.next_list) {
+      if (c.leader) {
+        const int slot = atomicAdd(p.next_count, 1);
+        p.next_list[slot] = pair;
+      }
+      return;
+    }
+    if (c.leader) {
+      p.status[pair] = c.spill ? kStCapacity : st;
+      p.wf_score[pair] = wf;
+      p.score[pair] = c.alg_score;
+      p.nruns[pair] = 0;
+    }
+    return;
+  }
+  uint32_t* base = p.runs + p.runs_off[pair];
+  const int t0 = K::kNW > 1 ? (int)threadIdx.x : c.lane;
+  for (int i = t0; i < plen + tlen + 1; i += K::kNW * kWave) base[i] = 0u;
+  if (c.leader) {
+    p.pend[pair] = 1;
+    p.pflag[pair] = 0;
+    p.pfail[pair] = ~0ull;
+    p.wf_score[pair] = INT_MIN;
+  }
+  __threadfence();
+  team_sync<K>();
+  const bool min_length = imax(plen, tlen) <= kFallbackMinLength;
+  run_task(c, p, Sub{pair, 0, plen, 0, tlen, CM, CM, min_length ? 0 : INT_MAX, 0}, staged, lds, wv);
+}
+
+// Work loop of a global tier in shared-task mode: queued sub-problems first,
+// then new pairs; ends when the list is drained, the queue is empty and no
+// aligner is still working (one that is may publish more).
+template <class K>
+DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) {
+  int32_t* q = p.qctl;
+  int staged = -1;
+  uint64_t last_progress = wall_clock64();
+  int last_sig = 0;   // queue counters last seen while idle
+  for (;;) {
+    int kind = 0, arg = 0;   // 1: queued item (arg = slot), 2: new pair (arg = id), -1: done
+    if (c.leader) {
+      for (;;) {
+        const int h = atomic_ld(&q[2]);
+        const int t = imin(atomic_ld(&q[3]), p.item_cap);
+        if (h < t) {
+          if (atomicCAS(&q[2], h, h + 1) == h) { kind = 1; arg = h; break; }
+          continue;
+        }
+        if (atomic_ld(&q[0]) < count) {
+          const int b = atomicAdd(&q[0], 1);
+          if (b < count) { kind = 2; arg = p.list[b]; break; }
+        }
+        const int busy = atomic_ld(&q[4]);
+        __threadfence();
+        const int h2 = atomic_ld(&q[2]);
+        const int t2 = imin(atomic_ld(&q[3]), p.item_cap);
+        kind = (busy == 0 && h2 >= t2) ? -1 : 0;
+        if (kind == 0) {
+          // idle: give up only when nothing at all has moved for kStallTicks
+          const int sig = h2 ^ (t2 << 11) ^ (busy << 22) ^ atomic_ld(&q[0]);
+          if (sig != last_sig) {
+            last_sig = sig;
+            last_progress = wall_clock64();
+          } else if (wall_clock64() - last_progress > kStallTicks) {
+            atomicOr(&q[5], 1);
+            kind = -1;
+          }
+        }
+        break;
+      }
+      if (kind > 0) atomicAdd(&q[4], 1);
+    }
+    team_bcast2(c, kind, arg);
+    if (kind < 0) break;
+    if (kind == 0) {
+      __builtin_amdgcn_s_sleep(20);
+      continue;
+    }
+    last_progress = wall_clock64();
+    if (kind == 1) {
+      int32_t* it = p.items + (int64_t)arg * kItemInts;
+      int ready = 1, unused = 0;
+      if (c.leader) {
+        while (atomic_ld(it) != p.epoch) {
+          if (wall_clock64() - last_progress > kStallTicks) { ready = 0; break; }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      team_bcast2(c, ready, unused);
+      if (!ready) {
+        if (c.leader) atomicOr(&q[5], 2);
+        break;
+      }
+      __threadfence();
+      const Sub s{uni(it[1]), uni(it[2]), uni(it[3]), uni(it[4]), uni(it[5]),
+                  uni(it[6]), uni(it[7]), uni(it[8]), uni(it[9])};
+      run_task(c, p, s, staged, lds, wv);
+    } else {
+      start_pair(c, p, arg, staged, lds, wv);
+    }
+    if (c.leader) {
+      __threadfence();
+      atomicSub(&q[4], 1);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1095,13 +1443,14 @@ void biwfa_persistent(KParams p) {
   c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
   c.gmask = p.gmask;
   const int count = uni(*(volatile const int32_t*)p.count);
+  if constexpr (!K::kLds) {
+    task_loop(c, p, count, lds, wv);
+  } else {
+  // LDS tier: one wave per pair, depth-first recursion on the wave alone
   for (;;) {
     int base = 0;
     if (c.leader) base = atomicAdd(p.queue, p.chunk);
     base = uni(__shfl(base, 0, kWave));
-    if constexpr (K::kNW > 1) {   // wave 0's value to the whole team
-      base = team_max1(c, wid == 0 ? base : INT_MIN);
-    }
     if (base >= count) break;
     const int end = imin(base + p.chunk, count);
     for (int w = base; w < end; ++w) {
@@ -1120,48 +1469,25 @@ void biwfa_persistent(KParams p) {
       c.out_n = 0; c.pend_op = -1; c.pend_len = 0; c.alg_score = 0;
       c.spill = 0; c.cap_err = 0; c.guard = 0;
       const int pw = (plen + 31) >> 5, tw = (tlen + 31) >> 5;
-      if (K::kLds) {
-        // stage both sequences: [pad][words...][pad][pad] (the pads keep the
-        // 64-bit windows in bounds at either end)
-        if (pw + 3 > p.L.seq_words || tw + 3 > p.L.seq_words) {
-          c.spill = 1;
-        } else {
-          uint64_t* sP = reinterpret_cast<uint64_t*>(ws + p.L.seq);
-          uint64_t* sT = sP + p.L.seq_words;
-          for (int i = lane; i < pw + 3; i += kWave) {
-            sP[i] = (i >= 1 && i <= pw) ? gP[i - 1] : 0ull;
-          }
-          for (int i = lane; i < tw + 3; i += kWave) {
-            sT[i] = (i >= 1 && i <= tw) ? gT[i - 1] : 0ull;
-          }
-          c.P = sP + 1;
-          c.T = sT + 1;
-          wave_sync<K::kLds>();
-        }
+      // stage both sequences: [pad][words...][pad][pad] (the pads keep the
+      // 64-bit windows in bounds at either end)
+      if (pw + 3 > p.L.seq_words || tw + 3 > p.L.seq_words) {
+        c.spill = 1;
       } else {
-        c.P = gP;
-        c.T = gT;
-        // the pair is staged in LDS when it fits (same padded layout): one area
-        // per team, or per wave for single-wave aligners
-        const int sw = p.seq_lds_words;
-        if (pw + 3 <= sw && tw + 3 <= sw) {
-          uint64_t* sP = reinterpret_cast<uint64_t*>(lds) + (K::kNW > 1 ? 0 : 2 * sw * wv);
-          uint64_t* sT = sP + sw;
-          const int t0 = K::kNW > 1 ? (int)threadIdx.x : lane;
-          team_sync<K>();   // the previous pair's readers are done
-          for (int i = t0; i < pw + 3; i += K::kNW * kWave) {
-            sP[i] = (i >= 1 && i <= pw) ? gP[i - 1] : 0ull;
-          }
-          for (int i = t0; i < tw + 3; i += K::kNW * kWave) {
-            sT[i] = (i >= 1 && i <= tw) ? gT[i - 1] : 0ull;
-          }
-          team_sync<K>();
-          c.P = sP + 1;
-          c.T = sT + 1;
+        uint64_t* sP = reinterpret_cast<uint64_t*>(ws + p.L.seq);
+        uint64_t* sT = sP + p.L.seq_words;
+        for (int i = lane; i < pw + 3; i += kWave) {
+          sP[i] = (i >= 1 && i <= pw) ? gP[i - 1] : 0ull;
         }
+        for (int i = lane; i < tw + 3; i += kWave) {
+          sT[i] = (i >= 1 && i <= tw) ? gT[i - 1] : 0ull;
+        }
+        c.P = sP + 1;
+        c.T = sT + 1;
+        wave_sync<K::kLds>();
       }
       int wf = INT_MIN, st = kStCapacity;
-      if (!c.spill) st = align_pair(c, plen, tlen, p.score_only, &wf);
+      if (!c.spill) st = align_pair(c, pair, plen, tlen, p.score_only, &wf);
       if (c.spill) {
         if (p.next_list) {      // re-run by the next tier with larger capacities
           if (c.leader) {
@@ -1180,6 +1506,7 @@ void biwfa_persistent(KParams p) {
         p.nruns[pair] = p.score_only ? 0 : c.out_n;
       }
     }
+  }
   }
 }
 

@@ -86,7 +86,7 @@ bw::WsLayout make_layout(int scope, int wcap, int arena_min, int hcap, int rcap,
   po += rcap;
   L.stack = po;
   L.scap = scap;
-  po += scap * 8;
+  po += scap * bw::kSubInts;
   L.priv_ints = (po + 15) & ~15;
   off += team_waves * L.priv_ints;
   L.seq = off;
@@ -169,7 +169,7 @@ struct bw_dbatch {
   int max_p = 0, max_t = 0, max_pt = 0;
   std::vector<Tier> tiers;
   int32_t* d_lists = nullptr;   // tiers x n pair ids
-  int32_t* d_init = nullptr;    // [2*kMaxTiers] counter initial values
+  int32_t* d_init = nullptr;    // [kQctlInts*kMaxTiers] counter initial values
   int32_t* d_score = nullptr;
   int32_t* d_wf = nullptr;
   int32_t* d_status = nullptr;
@@ -178,6 +178,12 @@ struct bw_dbatch {
   int64_t* d_runs_off = nullptr;
   int64_t runs_cap = 0;
   int32_t* d_gws = nullptr;
+  // shared recursion tasks of the global tiers
+  int32_t* d_items = nullptr;   // item_cap x kItemInts (ready flags start at 0)
+  int32_t item_cap = 0;
+  int32_t* d_pend = nullptr;
+  unsigned long long* d_pfail = nullptr;
+  int32_t* d_pflag = nullptr;
   int64_t gws_bytes = 0;
   std::vector<int32_t> tier_count;   // initial pairs per tier (last schedule)
 };
@@ -196,7 +202,8 @@ struct bw_aligner {
   float tier_ms[kMaxTiers] = {};
   int32_t tier_pairs[kMaxTiers] = {};   // pairs each tier took on (initial + spilled in)
   bw_tuning_t tuning{};
-  int32_t* d_counters = nullptr;   // per tier: {queue, count}
+  int32_t* d_counters = nullptr;   // per tier: {queue, count, head, tail, busy, error}
+  int32_t epoch_calls = 0;         // launches so far (ready-flag epochs)
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
   // last single-pair result
   int status = 0, score = 0, wf_score = 0;
@@ -273,7 +280,7 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
   a->lds_per_block_max = prop.sharedMemPerBlock;
   a->schedule = attr.schedule;
   if (hipSetDevice(a->dev) != hipSuccess || hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||
-      dalloc(&a->d_counters, 2 * kMaxTiers) != hipSuccess) {
+      dalloc(&a->d_counters, bw::kQctlInts * kMaxTiers) != hipSuccess) {
     set_error("stream / counter allocation failed");
     delete a;
     return nullptr;
@@ -301,6 +308,7 @@ void bw_dbatch_free(bw_dbatch_t* b) {
   dfree(b->d_words); dfree(b->d_pwoff); dfree(b->d_plen); dfree(b->d_twoff); dfree(b->d_tlen);
   dfree(b->d_lists); dfree(b->d_init); dfree(b->d_score); dfree(b->d_wf); dfree(b->d_status);
   dfree(b->d_nruns); dfree(b->d_runs); dfree(b->d_runs_off); dfree(b->d_gws);
+  dfree(b->d_items); dfree(b->d_pend); dfree(b->d_pfail); dfree(b->d_pflag);
   delete b;
 }
 
@@ -477,7 +485,13 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
   if ((e = dalloc(&b->d_twoff, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_tlen, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_lists, nt * (size_t)std::max<int64_t>(n, 1))) != hipSuccess) return fail(e);
-  if ((e = dalloc(&b->d_init, 2 * kMaxTiers)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_init, bw::kQctlInts * kMaxTiers)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_pend, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_pfail, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_pflag, n)) != hipSuccess) return fail(e);
+  b->item_cap = 1 << 20;
+  if ((e = dalloc(&b->d_items, (size_t)b->item_cap * bw::kItemInts)) != hipSuccess) return fail(e);
+  if ((e = hipMemsetAsync(b->d_items, 0, sizeof(int32_t) * (size_t)b->item_cap * bw::kItemInts, a->stream)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_score, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_wf, n)) != hipSuccess) return fail(e);
   if ((e = dalloc(&b->d_status, n)) != hipSuccess) return fail(e);
@@ -557,10 +571,10 @@ int bw_set_schedule(bw_aligner_t* a, bw_dbatch_t* b, int schedule, const float* 
   for (int32_t i : order) lists[tier[i]].push_back(i);
   a->schedule = schedule;
   HIP_TRY(hipSetDevice(a->dev));
-  std::vector<int32_t> init(2 * kMaxTiers, 0);
+  std::vector<int32_t> init(bw::kQctlInts * kMaxTiers, 0);
   b->tier_count.assign(nt, 0);
   for (size_t t = 0; t < nt; ++t) {
-    init[2 * t + 1] = (int32_t)lists[t].size();
+    init[bw::kQctlInts * t + 1] = (int32_t)lists[t].size();
     b->tier_count[t] = (int32_t)lists[t].size();
     if (!lists[t].empty())
       HIP_TRY(hipMemcpyAsync(b->d_lists + t * (size_t)n, lists[t].data(), 4 * lists[t].size(),
@@ -593,7 +607,9 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
   hipStream_t s = a->stream;
   const int nt = (int)b->tiers.size();
   HIP_TRY(hipEventRecord(a->ev[0], s));
-  hipLaunchKernelGGL(bw::biwfa_reset, dim3(1), dim3(64), 0, s, a->d_counters, b->d_init, 2 * kMaxTiers);
+  hipLaunchKernelGGL(bw::biwfa_reset, dim3(1), dim3(64), 0, s, a->d_counters, b->d_init,
+                     bw::kQctlInts * kMaxTiers);
+  ++a->epoch_calls;
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(a->ev[1], s));
   HIP_TRY(hipEventRecord(a->tev[0], s));
@@ -601,11 +617,19 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
     const Tier& T = b->tiers[t];
     bw::KParams p = base_params(a, b);
     p.list = b->d_lists + (size_t)t * b->n;
-    p.count = a->d_counters + 2 * t + 1;
-    p.queue = a->d_counters + 2 * t;
+    p.qctl = a->d_counters + bw::kQctlInts * t;
+    p.queue = p.qctl;
+    p.count = p.qctl + 1;
+    p.task = 1;
+    p.items = b->d_items;
+    p.item_cap = b->item_cap;
+    p.epoch = (int32_t)((((uint32_t)a->epoch_calls & 0x0fffffffu) << 3) | (uint32_t)t);
+    p.pend = b->d_pend;
+    p.pfail = b->d_pfail;
+    p.pflag = b->d_pflag;
     p.chunk = T.chunk;
     p.next_list = (t + 1 < nt) ? b->d_lists + (size_t)(t + 1) * b->n : nullptr;
-    p.next_count = (t + 1 < nt) ? a->d_counters + 2 * (t + 1) + 1 : nullptr;
+    p.next_count = (t + 1 < nt) ? a->d_counters + bw::kQctlInts * (t + 1) + 1 : nullptr;
     p.L = T.L;
     p.gws = b->d_gws;
     p.gws_stride = T.stride;
@@ -638,9 +662,16 @@ int bw_synchronize(bw_aligner_t* a) {
     float ms = 0.f;
     a->tier_ms[t] = hipEventElapsedTime(&ms, a->tev[t], a->tev[t + 1]) == hipSuccess ? ms : 0.f;
   }
-  int32_t cnt[2 * kMaxTiers];
+  int32_t cnt[bw::kQctlInts * kMaxTiers];
   HIP_TRY(hipMemcpy(cnt, a->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
-  for (int t = 0; t < kMaxTiers; ++t) a->tier_pairs[t] = t < a->last_ntiers ? cnt[2 * t + 1] : 0;
+  for (int t = 0; t < kMaxTiers; ++t) a->tier_pairs[t] = t < a->last_ntiers ? cnt[bw::kQctlInts * t + 1] : 0;
+  for (int t = 0; t < a->last_ntiers; ++t) {
+    if (cnt[bw::kQctlInts * t + 5] != 0) {
+      set_error("tier " + std::to_string(t) + ": a shared recursion task made no progress (device flag " +
+                std::to_string(cnt[bw::kQctlInts * t + 5]) + ")");
+      return BW_STATUS_INTERNAL;
+    }
+  }
   return 0;
 }
 
