@@ -164,6 +164,7 @@ def main():
     aligner = bw.WavefrontAligner(penalties=bw.configs.PENALTIES, schedule=sched)
     if args.tuning:
         aligner.tuning = json.loads(args.tuning)
+    sample = None
     if args.autotune > 0:
         t0 = time.time()
         sample = bw.generate_dataset(cfg, num_pairs=min(args.autotune, n), seed=cfg.seed + 1000).pack()
@@ -195,11 +196,19 @@ def main():
         from biwfa_gpu_pii2025_amd import costmodel as cm
         plain_ms = timed("plain: ")[0]
         plain_res = db.download()
+        plain_tuning = aligner.tuning
+        db.free()
         t0 = time.time()
         model, tinfo = cm.train_cost_model(num_pairs=args.train_pairs, epochs=30, seed=0)
         train_s = time.time() - t0
         log("cost model trained on %d aligner-labelled pairs in %.1f s (mse %.5f -> %.5f)" % (
             tinfo["pairs"], train_s, tinfo["initial_mse"], tinfo["final_mse"]))
+        if sample is not None:
+            t0 = time.time()
+            spred = model.predict_score(cm.batch_features(sample)).astype(np.float32)
+            tuned = aligner.autotune(sample, repeats=2, predicted_score=spred)
+            log("learned schedule autotuned in %.1f s: %s" % (time.time() - t0, json.dumps(tuned)))
+        db = aligner.upload(packed)
         t0 = time.time()
         pred = cm.learned_schedule(db, model, packed)
         sched_s = time.time() - t0
@@ -209,6 +218,7 @@ def main():
         rank_t = np.argsort(np.argsort(true_score))
         spearman = float(np.corrcoef(rank_p, rank_t)[0, 1])
         sched_info = {"plain_ms_per_step": round(plain_ms, 4), "plain_pairs_s": round(n / plain_ms * 1e3, 1),
+                      "plain_tuning": plain_tuning,
                       "train_pairs": tinfo["pairs"], "train_s": round(train_s, 2),
                       "train_mse": [round(tinfo["initial_mse"], 6), round(tinfo["final_mse"], 6)],
                       "predict_and_schedule_s": round(sched_s, 3),

@@ -124,7 +124,8 @@ def lib():
         L.bw_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Tuning)]
         L.bw_last_tier_timing.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.bw_last_tier_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
-        L.bw_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch), ctypes.c_int, ctypes.POINTER(_Tuning)]
+        L.bw_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch), ctypes.c_void_p, ctypes.c_int,
+                                  ctypes.POINTER(_Tuning)]
         L.bw_dataset_lengths.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_generate.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]
@@ -450,11 +451,17 @@ class WavefrontAligner:
         if st != 0:
             raise BiwfaError("bw_set_tuning: " + last_error())
 
-    def autotune(self, packed, repeats=2):
-        """Times the sample under candidate tunings (bw_autotune); keeps and returns the best."""
+    def autotune(self, packed, repeats=2, predicted_score=None):
+        """Times the sample under candidate tunings (bw_autotune); keeps and returns the best.
+        With predicted scores (one per sample pair) the sample runs under the learned schedule."""
         b = packed._struct()
         t = _Tuning()
-        st = self._L.bw_autotune(self._h, ctypes.byref(b), int(repeats), ctypes.byref(t))
+        pred = None
+        if predicted_score is not None:
+            pred = np.ascontiguousarray(predicted_score, np.float32)
+            if len(pred) != len(packed):
+                raise BiwfaError("one predicted score per sample pair is needed")
+        st = self._L.bw_autotune(self._h, ctypes.byref(b), _ptr(pred), int(repeats), ctypes.byref(t))
         if st != 0:
             raise BiwfaError("bw_autotune: %s (%s)" % (strerror(st), last_error()))
         return self.tuning

@@ -788,12 +788,16 @@ int bw_last_tier_pairs(const bw_aligner_t* a, int32_t* pairs, int capacity) {
 
 // Runs the sample under tuning `t` (upload, one warm-up, `repeats` timed runs)
 // and returns the mean per-tier and total milliseconds.
-static int time_tuning(bw_aligner_t* a, const bw_batch_t* sample, const bw_tuning_t& t, int repeats,
-                       std::vector<float>* tier_ms, float* total_ms) {
+static int time_tuning(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, const bw_tuning_t& t,
+                       int repeats, std::vector<float>* tier_ms, float* total_ms) {
   a->tuning = t;
   bw_dbatch_t* b = nullptr;
   int st = upload_impl(a, sample, &b);
   if (st != 0) return st;
+  if (pred) {
+    st = bw_set_schedule(a, b, BW_SCHEDULE_LEARNED, pred);
+    if (st != 0) { bw_dbatch_free(b); return st; }
+  }
   tier_ms->assign(kMaxTiers, 0.f);
   *total_ms = 0.f;
   for (int r = 0; r <= repeats && st == 0; ++r) {
@@ -807,8 +811,10 @@ static int time_tuning(bw_aligner_t* a, const bw_batch_t* sample, const bw_tunin
   return st;
 }
 
-int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, int repeats, bw_tuning_t* best) {
+int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, int repeats,
+                bw_tuning_t* best) {
   if (!a || !sample) return BW_STATUS_INVALID_ARG;
+  const int sched0 = a->schedule;
   if (repeats < 1) repeats = 1;
   const bw_tuning_t start = a->tuning;
   bw_tuning_t cur = start;
@@ -823,8 +829,8 @@ int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, int repeats, bw_tunin
   for (int nw : kTeams) {
     bw_tuning_t t = cur;
     for (int i = 0; i < BW_MAX_TIERS; ++i) t.team_waves[i] = nw;
-    int st = time_tuning(a, sample, t, repeats, &tms, &total);
-    if (st != 0) { a->tuning = start; return st; }
+    int st = time_tuning(a, sample, pred, t, repeats, &tms, &total);
+    if (st != 0) { a->tuning = start; a->schedule = sched0; return st; }
     for (int i = 1; i < kMaxTiers; ++i) {
       if (tms[i] < best_tier[i]) {
         best_tier[i] = tms[i];
@@ -841,8 +847,8 @@ int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, int repeats, bw_tunin
   for (int ch : kChunks) {
     bw_tuning_t t = cur;
     t.queue_chunk[0] = ch;
-    int st = time_tuning(a, sample, t, repeats, &tms, &total);
-    if (st != 0) { a->tuning = start; return st; }
+    int st = time_tuning(a, sample, pred, t, repeats, &tms, &total);
+    if (st != 0) { a->tuning = start; a->schedule = sched0; return st; }
     if (total < best_total) { best_total = total; pick = t; }
   }
   cur = pick;
@@ -850,11 +856,12 @@ int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, int repeats, bw_tunin
   for (int w : kWidths) {
     bw_tuning_t t = cur;
     t.lds_ring_width = w;
-    int st = time_tuning(a, sample, t, repeats, &tms, &total);
-    if (st != 0) { a->tuning = start; return st; }
+    int st = time_tuning(a, sample, pred, t, repeats, &tms, &total);
+    if (st != 0) { a->tuning = start; a->schedule = sched0; return st; }
     if (total < best_total) { best_total = total; pick = t; }
   }
   a->tuning = pick;
+  a->schedule = sched0;
   if (best) *best = pick;
   return 0;
 }

@@ -187,8 +187,9 @@ void bw_dbatch_free(bw_dbatch_t* dbatch);
 
 /* Milliseconds of the last bw_dbatch_align, measured with HIP events on the
  * aligner's stream (valid after bw_synchronize): *total_ms the whole launch
- * sequence, *main_ms the first persistent kernel (LDS workspace, every pair),
- * *spill_ms the second one (global workspace, pairs the first one spilled). */
+ * sequence, *main_ms the LDS tier (every pair that fits it), *spill_ms the
+ * global tiers, run in order of ring width over their own pairs and the pairs
+ * the tier below passed on. */
 int bw_last_timing(const bw_aligner_t* aligner, float* total_ms, float* main_ms, float* spill_ms);
 
 /* ---------------- (iv) scheduling ---------------- */
@@ -234,9 +235,12 @@ int bw_last_tier_pairs(const bw_aligner_t* aligner, int32_t* pairs, int capacity
 
 /* Times the sample batch under candidate tunings and keeps the fastest: team
  * waves per global tier (each tier timed on its own), then the LDS tier's queue
- * chunk and ring width (whole batch).  `repeats` timed runs per candidate (after
- * one warm-up).  The result is set on the aligner and copied to *best. */
-int bw_autotune(bw_aligner_t* aligner, const bw_batch_t* sample, int repeats, bw_tuning_t* best);
+ * chunk and ring width (whole batch).  The sample runs under the aligner's
+ * schedule, or the learned one when `predicted_score` (one per sample pair) is
+ * given.  `repeats` timed runs per candidate (after one warm-up).  The result is
+ * set on the aligner and copied to *best. */
+int bw_autotune(bw_aligner_t* aligner, const bw_batch_t* sample, const float* predicted_score,
+                int repeats, bw_tuning_t* best);
 
 /* ---------------- (v) learned cost model ---------------- */
 
