@@ -168,7 +168,7 @@ def main():
     if args.autotune > 0:
         t0 = time.time()
         sample = bw.generate_dataset(cfg, num_pairs=min(args.autotune, n), seed=cfg.seed + 1000).pack()
-        tuned = aligner.autotune(sample, repeats=2)
+        tuned = aligner.autotune(sample, repeats=2, progress=log)
         log("autotuned on %d pairs in %.1f s: %s" % (len(sample), time.time() - t0, json.dumps(tuned)))
     db = aligner.upload(packed)
 
@@ -206,7 +206,7 @@ def main():
         if sample is not None:
             t0 = time.time()
             spred = model.predict_score(cm.batch_features(sample)).astype(np.float32)
-            tuned = aligner.autotune(sample, repeats=2, predicted_score=spred)
+            tuned = aligner.autotune(sample, repeats=2, predicted_score=spred, progress=log)
             log("learned schedule autotuned in %.1f s: %s" % (time.time() - t0, json.dumps(tuned)))
         db = aligner.upload(packed)
         t0 = time.time()

@@ -68,6 +68,7 @@ class _Result(ctypes.Structure):
 
 
 MAX_TIERS = 8
+_PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
 
 
 class _Tuning(ctypes.Structure):
@@ -126,6 +127,7 @@ def lib():
         L.bw_last_tier_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.bw_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch), ctypes.c_void_p, ctypes.c_int,
                                   ctypes.POINTER(_Tuning)]
+        L.bw_set_progress.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_lengths.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_generate.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]
@@ -451,9 +453,14 @@ class WavefrontAligner:
         if st != 0:
             raise BiwfaError("bw_set_tuning: " + last_error())
 
-    def autotune(self, packed, repeats=2, predicted_score=None):
+    def autotune(self, packed, repeats=2, predicted_score=None, progress=None):
         """Times the sample under candidate tunings (bw_autotune); keeps and returns the best.
-        With predicted scores (one per sample pair) the sample runs under the learned schedule."""
+        With predicted scores (one per sample pair) the sample runs under the learned schedule.
+        `progress(str)` receives one line per candidate."""
+        cb = None
+        if progress is not None:
+            cb = _PROGRESS_FN(lambda msg, user: progress(msg.decode()))
+        self._L.bw_set_progress(self._h, cb, None)
         b = packed._struct()
         t = _Tuning()
         pred = None
@@ -462,6 +469,7 @@ class WavefrontAligner:
             if len(pred) != len(packed):
                 raise BiwfaError("one predicted score per sample pair is needed")
         st = self._L.bw_autotune(self._h, ctypes.byref(b), _ptr(pred), int(repeats), ctypes.byref(t))
+        self._L.bw_set_progress(self._h, None, None)
         if st != 0:
             raise BiwfaError("bw_autotune: %s (%s)" % (strerror(st), last_error()))
         return self.tuning

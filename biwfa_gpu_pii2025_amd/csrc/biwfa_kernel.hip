@@ -380,6 +380,30 @@ DI void null_step(Ctx<K>& c, Dir& d, int s, int slot) {
   set_hdr(c, d, s, slot, CD, kAbsent, 0, 0);
 }
 
+// The four input wavefronts of a base-case step held in memory (global
+// workspace): lane j loads the header of input j -- M[s-x], M[s-o-e], I1[s-e],
+// D1[s-e] -- so the four header loads overlap.  Same result as get_in.
+template <class K>
+DI void base_inputs(const Ctx<K>& c, int s, Wf& mm, Wf& mo, Wf& ie, Wf& de) {
+  const int l = c.lane;
+  const int sc = s - ((l == 0) ? c.x : (l == 1) ? c.o + c.e : c.e);
+  const int comp = (l <= 1) ? CM : (l == 2) ? CI : CD;
+  int lo = kAbsent, hi = 0, kb = 0;
+  if (l < 4 && sc >= 0) {
+    const int h = c.L.bhdr + (sc * 3 + comp) * 3;
+    lo = c.pv[h];
+    hi = c.pv[h + 1];
+    kb = c.pv[h + 2];
+  }
+  Wf w[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int wl = rdl(lo, j), wh = rdl(hi, j);
+    w[j] = (wl == kAbsent || wl > wh) ? wf_null() : Wf{wl, wh, rdl(kb, j), -1};
+  }
+  mm = w[0]; mo = w[1]; ie = w[2]; de = w[3];
+}
+
 // One WFA step at score s: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
 // followed by wavefront_extend_end2end(_max) (wavefront_extend.c:86-211) and the
 // termination test (wavefront_termination.c:37-114), fused: the lane that
@@ -400,10 +424,14 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   // reachable scores: the other steps are null without looking at the inputs.
   Wf mm, mo, ie, de;
   if (!null) {
-    mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
-    mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
-    ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
-    de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
+    if (!K::kLds && d.ring < 0) {
+      base_inputs(c, s, mm, mo, ie, de);
+    } else {
+      mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
+      mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
+      ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
+      de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
+    }
     null = mm.lo > mm.hi && mo.lo > mo.hi && ie.lo > ie.hi && de.lo > de.hi;
   }
   if (null) {                                   // null step (wavefront_compute_affine.c:245-251)
@@ -673,12 +701,15 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
   const Wf i0 = get_wf(c, a0, s0, slot0, CI, &pi0);
   const Wf d0 = get_wf(c, a0, s0, slot0, CD, &pd0);
   const int tl = a0.tlen, pl = a0.plen;
+  // ring slots of a1 holding any component (null scores hold none)
+  const uint64_t live1 = ((c.rpres[0] | c.rpres[1] | c.rpres[2]) >> (32 * a1.ring)) & 0xffffffffull;
   if constexpr (K::kNW == 1) {
     for (int i = 0; i < c.scope; ++i) {
       const int si = s1 - i;
       if (si < 0) break;
       if (s0 + si - c.o >= bp.score) continue;
       const int sl = slot_back(slot1, i, c.scope);
+      if (!((live1 >> sl) & 1ull)) continue;
       int p1;
       const Wf d1 = get_wf(c, a1, si, sl, CD, &p1);
       if (pd0 && p1) scan_breakpoint(c, forward, s0, si, d0, d1, CD, tl, pl, bp);
@@ -699,8 +730,9 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
     for (int i = 0; i < c.scope; ++i) {
       const int si = s1 - i;
       int hd = INT_MAX, hi = INT_MAX, hm = INT_MAX;
-      if (si >= 0 && s0 + si - c.o < entry) {
-        const int sl = slot_back(slot1, i, c.scope);
+      const int sl0 = slot_back(slot1, i, c.scope);
+      if (si >= 0 && s0 + si - c.o < entry && ((live1 >> sl0) & 1ull)) {
+        const int sl = sl0;
         int p1;
         const Wf d1 = get_wf(c, a1, si, sl, CD, &p1);
         if (pd0 && p1) hd = scan_first(c, forward, d0, d1, CD, tl, pl);
@@ -723,6 +755,7 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
       if (si < 0) break;
       if (s0 + si - c.o >= bp.score) continue;
       const int sl = slot_back(slot1, i, c.scope);
+      if (!((live1 >> sl) & 1ull)) continue;
       int p1;
       const Wf d1 = get_wf(c, a1, si, sl, CD, &p1);
       if (pd0 && p1) accept_hit(c, buf, 3 * i, forward, s0, si, d0, d1, CD, tl, pl, bp);

@@ -202,6 +202,8 @@ struct bw_aligner {
   float tier_ms[kMaxTiers] = {};
   int32_t tier_pairs[kMaxTiers] = {};   // pairs each tier took on (initial + spilled in)
   bw_tuning_t tuning{};
+  bw_progress_fn progress = nullptr;
+  void* progress_user = nullptr;
   int32_t* d_counters = nullptr;   // per tier: {queue, count, head, tail, busy, error}
   int32_t epoch_calls = 0;         // launches so far (ready-flag epochs)
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
@@ -811,6 +813,25 @@ static int time_tuning(bw_aligner_t* a, const bw_batch_t* sample, const float* p
   return st;
 }
 
+int bw_set_progress(bw_aligner_t* a, bw_progress_fn fn, void* user) {
+  if (!a) return BW_STATUS_INVALID_ARG;
+  a->progress = fn;
+  a->progress_user = user;
+  return 0;
+}
+
+static void report(const bw_aligner_t* a, const std::string& msg) {
+  if (a->progress) a->progress(msg.c_str(), a->progress_user);
+}
+
+// Smallest team that keeps a wave at no more than 64 chunks of 64 diagonals per
+// step on a ring of width wcap (fewer lanes only make such a tier crawl).
+static int min_team_for(int wcap) {
+  int nw = 1;
+  while (nw < 16 && wcap > 64 * 64 * nw) nw *= 2;
+  return nw;
+}
+
 int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, int repeats,
                 bw_tuning_t* best) {
   if (!a || !sample) return BW_STATUS_INVALID_ARG;
@@ -828,11 +849,19 @@ int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, in
   bw_tuning_t pick = cur;
   for (int nw : kTeams) {
     bw_tuning_t t = cur;
-    for (int i = 0; i < BW_MAX_TIERS; ++i) t.team_waves[i] = nw;
+    bool allowed[BW_MAX_TIERS];
+    for (int i = 0; i < BW_MAX_TIERS; ++i) {
+      const int lo = min_team_for(256 << (2 * i));
+      allowed[i] = nw >= lo;
+      t.team_waves[i] = std::max(nw, lo);
+    }
     int st = time_tuning(a, sample, pred, t, repeats, &tms, &total);
     if (st != 0) { a->tuning = start; a->schedule = sched0; return st; }
+    std::string line = "autotune team_waves=" + std::to_string(nw) + ": total " + std::to_string(total) + " ms, tiers";
+    for (int i = 0; i < a->last_ntiers; ++i) line += " " + std::to_string(tms[i]);
+    report(a, line);
     for (int i = 1; i < kMaxTiers; ++i) {
-      if (tms[i] < best_tier[i]) {
+      if (allowed[i - 1] && tms[i] < best_tier[i]) {
         best_tier[i] = tms[i];
         pick.team_waves[i - 1] = nw;
       }
@@ -849,6 +878,7 @@ int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, in
     t.queue_chunk[0] = ch;
     int st = time_tuning(a, sample, pred, t, repeats, &tms, &total);
     if (st != 0) { a->tuning = start; a->schedule = sched0; return st; }
+    report(a, "autotune lds queue_chunk=" + std::to_string(ch) + ": total " + std::to_string(total) + " ms");
     if (total < best_total) { best_total = total; pick = t; }
   }
   cur = pick;
@@ -858,6 +888,7 @@ int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, in
     t.lds_ring_width = w;
     int st = time_tuning(a, sample, pred, t, repeats, &tms, &total);
     if (st != 0) { a->tuning = start; a->schedule = sched0; return st; }
+    report(a, "autotune lds_ring_width=" + std::to_string(w) + ": total " + std::to_string(total) + " ms");
     if (total < best_total) { best_total = total; pick = t; }
   }
   a->tuning = pick;

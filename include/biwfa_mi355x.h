@@ -242,6 +242,11 @@ int bw_last_tier_pairs(const bw_aligner_t* aligner, int32_t* pairs, int capacity
 int bw_autotune(bw_aligner_t* aligner, const bw_batch_t* sample, const float* predicted_score,
                 int repeats, bw_tuning_t* best);
 
+/* Optional progress reports of long calls (bw_autotune: one line per candidate
+ * timed).  fn may be NULL to switch them off. */
+typedef void (*bw_progress_fn)(const char* message, void* user);
+int bw_set_progress(bw_aligner_t* aligner, bw_progress_fn fn, void* user);
+
 /* ---------------- (v) learned cost model ---------------- */
 
 /* No counterpart in the reference: a 4 -> 64 -> 64 -> 1 ReLU MLP (BF16 weights,
