@@ -480,7 +480,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   // far as the compiler knows, so without the grouping every chunk would wait
   // out a full memory latency on its own.  (The LDS ring is fast enough and
   // narrow enough for one chunk at a time.)
-  constexpr int kG = K::kLds ? 1 : 3;
+  constexpr int kG = K::kLds ? 1 : 2;
   constexpr int kStride = K::kNW * kWave;
   for (int kb0 = lo + c.wid * kWave; kb0 <= hi; kb0 += kG * kStride) {
     int ins1[kG], del1[kG], misms[kG];
@@ -1146,9 +1146,14 @@ DI int align_pair(Ctx<K>& c, int pair, int plen, int tlen, int score_only, int* 
 //     the flag and fences before reading.
 
 constexpr int kFlagSpill = 1, kFlagRuns = 2;
-// an aligner that sees no progress for this long gives up (a lost item can
-// only be a defect); 100 MHz constant clock
-constexpr uint64_t kStallTicks = 20ull * 100000000ull;
+// A claimed item's producer writes it within microseconds of claiming its slot;
+// a taker that waits longer than this for the ready flag gives up and raises
+// the tier's error flag (a defect, never seen in a correct run).  100 MHz clock.
+constexpr uint64_t kReadyTicks = 60ull * 100000000ull;
+// Pairs shorter than this (p + t) are aligned by one aligner from start to end
+// with the runs emitted in order: sharing their halves costs more (zeroing,
+// merging, agent-scope fences) than it saves.
+constexpr int kTaskMinLength = 4096;
 
 DI int atomic_ld(const int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
@@ -1355,6 +1360,36 @@ DI void start_pair(Ctx<K>& c, const KParams& p, int pair, int& staged, int32_t* 
     }
     return;
   }
+  if (plen + tlen < kTaskMinLength) {   // short pair: depth first on this aligner
+    if (pair != staged) {
+      stage_pair(c, p, pair, lds, wv);
+      staged = pair;
+    }
+    c.out = p.runs + p.runs_off[pair];
+    c.out_cap = plen + tlen + 1;
+    c.out_n = 0; c.pend_op = -1; c.pend_len = 0; c.alg_score = 0;
+    c.spill = 0; c.cap_err = 0; c.arena_top = 0;
+    int wf = INT_MIN;
+    int st = align_pair(c, pair, plen, tlen, 0, &wf);
+    if (c.spill) {
+      if (p.next_list) {
+        if (c.leader) {
+          const int slot = atomicAdd(p.next_count, 1);
+          p.next_list[slot] = pair;
+        }
+        return;
+      }
+      st = kStCapacity;
+    }
+    if (c.cap_err) st = kStCapacity;
+    if (c.leader) {
+      p.status[pair] = st;
+      p.wf_score[pair] = wf;
+      p.score[pair] = (st == 0 && c.out_n > 0) ? c.alg_score : -1;
+      p.nruns[pair] = st == 0 ? c.out_n : 0;
+    }
+    return;
+  }
   uint32_t* base = p.runs + p.runs_off[pair];
   const int t0 = K::kNW > 1 ? (int)threadIdx.x : c.lane;
   for (int i = t0; i < plen + tlen + 1; i += K::kNW * kWave) base[i] = 0u;
@@ -1377,8 +1412,6 @@ template <class K>
 DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) {
   int32_t* q = p.qctl;
   int staged = -1;
-  uint64_t last_progress = wall_clock64();
-  int last_sig = 0;   // queue counters last seen while idle
   for (;;) {
     int kind = 0, arg = 0;   // 1: queued item (arg = slot), 2: new pair (arg = id), -1: done
     if (c.leader) {
@@ -1398,17 +1431,6 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
         const int h2 = atomic_ld(&q[2]);
         const int t2 = imin(atomic_ld(&q[3]), p.item_cap);
         kind = (busy == 0 && h2 >= t2) ? -1 : 0;
-        if (kind == 0) {
-          // idle: give up only when nothing at all has moved for kStallTicks
-          const int sig = h2 ^ (t2 << 11) ^ (busy << 22) ^ atomic_ld(&q[0]);
-          if (sig != last_sig) {
-            last_sig = sig;
-            last_progress = wall_clock64();
-          } else if (wall_clock64() - last_progress > kStallTicks) {
-            atomicOr(&q[5], 1);
-            kind = -1;
-          }
-        }
         break;
       }
       if (kind > 0) atomicAdd(&q[4], 1);
@@ -1419,13 +1441,13 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
       __builtin_amdgcn_s_sleep(20);
       continue;
     }
-    last_progress = wall_clock64();
     if (kind == 1) {
       int32_t* it = p.items + (int64_t)arg * kItemInts;
       int ready = 1, unused = 0;
       if (c.leader) {
+        const uint64_t t0 = wall_clock64();
         while (atomic_ld(it) != p.epoch) {
-          if (wall_clock64() - last_progress > kStallTicks) { ready = 0; break; }
+          if (wall_clock64() - t0 > kReadyTicks) { ready = 0; break; }
           __builtin_amdgcn_s_sleep(2);
         }
       }

@@ -332,14 +332,14 @@ static int base_score_bound(const Penalties& q) {
   return std::max(bw::kRecoveryMinScore, 100 * q.x + q.o + 100 * q.e) + 1;
 }
 
-// Waves per aligner of the global tier with ring width `wcap`: one wave while
-// a wavefront spans a few 64-diagonal chunks, then teams sized so that each
-// wave keeps about 2-4 chunks per step.
+// Waves per aligner of the global tier with ring width `wcap` (the built-in
+// rule, from bw_autotune runs on the mixed-length, long-pair and 1 kbp batches:
+// a step is bound by the latency of its HBM round trips, so wider teams pay
+// off as soon as a wavefront spans several chunks).
 static int team_waves(const bw_aligner_t* a, int wcap) {
   (void)a;
-  if (wcap <= 256) return 1;
-  if (wcap <= 1024) return 4;
-  if (wcap <= 4096) return 8;
+  if (wcap <= 256) return 4;
+  if (wcap <= 1024) return 8;
   return 16;
 }
 
