@@ -28,7 +28,7 @@ constexpr int kMaxTeam = 16;           // waves cooperating on one pair (one wor
 constexpr int kRedSlots = 3 * kMaxScope;   // team reduction values per wave
 constexpr int kSubInts = 10;           // recursion stack entry (a sub-problem)
 constexpr int kItemInts = 12;          // shared recursion task: ready flag + sub-problem
-constexpr int kQctlInts = 6;           // per tier: {queue, count, head, tail, busy, error}
+constexpr int kQctlInts = 6;           // per tier: {queue, count, head, tail, in-flight task pairs, error}
 
 // Kernel configuration: kLds keeps the whole workspace and both sequences in
 // LDS (one wave per pair); otherwise the workspace is a global slab shared by a
@@ -104,7 +104,7 @@ struct KParams {
   int32_t* items;                // item_cap x kItemInts
   int32_t item_cap;
   int32_t epoch;                 // ready-flag value of this launch (never 0)
-  int32_t* qctl;                 // this tier's {queue, count, head, tail, busy, error}
+  int32_t* qctl;                 // this tier's {queue, count, head, tail, in-flight task pairs, error}
   int32_t* pend;                 // per pair: unfinished sub-problems
   unsigned long long* pfail;     // per pair: leftmost failure, (pb+tb) << 16 | -status
   int32_t* pflag;                // per pair: 1 workspace capacity exceeded, 2 run overflow

@@ -900,6 +900,44 @@ DI long long bt_src(const Ctx<K>& c, const Dir& b, int score, int comp, int k, i
 }
 DI long long max64(long long a, long long b) { return a >= b ? a : b; }
 
+// The packed maximum of bt_src over the sources of (matrix, score, k) when the
+// base-case headers live in memory (global workspace): the five header reads
+// are issued together, then the five offset reads, with no branch in between,
+// so a backtrace step costs two memory round trips instead of up to ten.
+template <class K>
+DI long long bt_max_mem(const Ctx<K>& c, int score, int matrix, int k) {
+  const int sj[5] = {score - c.x, score - c.o - c.e, score - c.e, score - c.o - c.e, score - c.e};
+  const int cj[5] = {CM, CM, CI, CM, CD};
+  const int dk[5] = {0, -1, -1, 1, 1};
+  const int ad[5] = {1, 1, 1, 0, 0};
+  const int ty[5] = {kBtM, kBtI1Open, kBtI1Ext, kBtD1Open, kBtD1Ext};
+  const bool en[5] = {matrix == CM, matrix != CD, matrix != CD, matrix != CI, matrix != CI};
+  int lo[5], hi[5], kb[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int h = c.L.bhdr + ((sj[j] >= 0 ? sj[j] : 0) * 3 + cj[j]) * 3;
+    lo[j] = c.pv[h];
+    hi[j] = c.pv[h + 1];
+    kb[j] = c.pv[h + 2];
+  }
+  bool in[5];
+  int off[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int l = uni(lo[j]), u = uni(hi[j]), kk = k + dk[j];
+    in[j] = en[j] && sj[j] >= 0 && l != kAbsent && l <= kk && kk <= u;
+    off[j] = c.ws[in[j] ? uni(kb[j]) + kk : c.L.data];
+  }
+  long long m = LLONG_MIN;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    if (!en[j]) continue;
+    const long long v = in[j] ? (((long long)(uni(off[j]) + ad[j]) << 4) | ty[j]) : (long long)kOffsetNull;
+    m = max64(m, v);
+  }
+  return m;
+}
+
 // wavefront_backtrace_affine (wavefront_backtrace.c:320-529); appends the
 // operations to the output in forward order.  Returns 0 or -1 on a trace error.
 template <class K>
@@ -917,7 +955,9 @@ DI int backtrace(Ctx<K>& c, const Dir& b, int score) {
     const int gap_open1 = score - c.o - c.e;
     const int gap_extend1 = score - c.e;
     long long max_all;
-    if (matrix == CM) {
+    if constexpr (!K::kLds) {
+      max_all = bt_max_mem(c, score, matrix, k);
+    } else if (matrix == CM) {
       const long long misms = bt_src(c, b, mismatch, CM, k, 1, kBtM);
       const long long io = bt_src(c, b, gap_open1, CM, k - 1, 1, kBtI1Open);
       const long long ix = bt_src(c, b, gap_extend1, CI, k - 1, 1, kBtI1Ext);
@@ -968,9 +1008,21 @@ DI int backtrace(Ctx<K>& c, const Dir& b, int score) {
   wave_sync<K::kLds>();
   // forward order: the pending run first, then the stored runs backwards
   out_append(c, rr.op, rr.len);
-  for (int i = rr.n - 1; i >= 0; --i) {
-    const uint32_t run = (uint32_t)uni(c.pv[c.L.runs_tmp + i]);
-    out_append(c, (int)(run & 15u), (int)(run >> 4));
+  if constexpr (K::kLds) {
+    for (int i = rr.n - 1; i >= 0; --i) {
+      const uint32_t run = (uint32_t)uni(c.pv[c.L.runs_tmp + i]);
+      out_append(c, (int)(run & 15u), (int)(run >> 4));
+    }
+  } else {   // 64 runs per memory round trip
+    for (int i0 = rr.n - 1; i0 >= 0; i0 -= kWave) {
+      const int i = i0 - c.lane;
+      const int v = i >= 0 ? c.pv[c.L.runs_tmp + i] : 0;
+      const int cnt = imin(kWave, i0 + 1);
+      for (int j = 0; j < cnt; ++j) {
+        const uint32_t run = (uint32_t)rdl(v, j);
+        out_append(c, (int)(run & 15u), (int)(run >> 4));
+      }
+    }
   }
   return 0;
 }
@@ -1229,6 +1281,7 @@ DI void finish_pair(Ctx<K>& c, const KParams& p, int pair) {
       if (c.leader) {
         const int slot = atomicAdd(p.next_count, 1);
         p.next_list[slot] = pair;
+        atomicSub(&p.qctl[4], 1);   // no longer in flight here
       }
       return;
     }
@@ -1260,6 +1313,7 @@ DI void finish_pair(Ctx<K>& c, const KParams& p, int pair) {
     p.status[pair] = st;
     p.score[pair] = (st == 0 && c.out_n > 0) ? c.alg_score : -1;
     p.nruns[pair] = st == 0 ? c.out_n : 0;
+    atomicSub(&p.qctl[4], 1);   // no longer in flight
   }
   team_sync<K>();
 }
@@ -1394,6 +1448,7 @@ DI void start_pair(Ctx<K>& c, const KParams& p, int pair, int& staged, int32_t* 
   const int t0 = K::kNW > 1 ? (int)threadIdx.x : c.lane;
   for (int i = t0; i < plen + tlen + 1; i += K::kNW * kWave) base[i] = 0u;
   if (c.leader) {
+    atomicAdd(&p.qctl[4], 1);   // a shared-task pair in flight: it may publish items
     p.pend[pair] = 1;
     p.pflag[pair] = 0;
     p.pfail[pair] = ~0ull;
@@ -1407,11 +1462,13 @@ DI void start_pair(Ctx<K>& c, const KParams& p, int pair, int& staged, int32_t* 
 
 // Work loop of a global tier in shared-task mode: queued sub-problems first,
 // then new pairs; ends when the list is drained, the queue is empty and no
-// aligner is still working (one that is may publish more).
+// shared-task pair is in flight (only those publish items; the in-flight count
+// drops after a pair's last sub-problem, which published nothing more).
 template <class K>
 DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) {
   int32_t* q = p.qctl;
   int staged = -1;
+  int idle = 0;
   for (;;) {
     int kind = 0, arg = 0;   // 1: queued item (arg = slot), 2: new pair (arg = id), -1: done
     if (c.leader) {
@@ -1426,21 +1483,24 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
           const int b = atomicAdd(&q[0], 1);
           if (b < count) { kind = 2; arg = p.list[b]; break; }
         }
-        const int busy = atomic_ld(&q[4]);
+        // list drained: more work can only come from shared-task pairs still in
+        // flight (their items); with none, this aligner is done
+        const int inflight = atomic_ld(&q[4]);
         __threadfence();
         const int h2 = atomic_ld(&q[2]);
         const int t2 = imin(atomic_ld(&q[3]), p.item_cap);
-        kind = (busy == 0 && h2 >= t2) ? -1 : 0;
+        kind = (inflight == 0 && h2 >= t2) ? -1 : 0;
         break;
       }
-      if (kind > 0) atomicAdd(&q[4], 1);
     }
     team_bcast2(c, kind, arg);
     if (kind < 0) break;
-    if (kind == 0) {
-      __builtin_amdgcn_s_sleep(20);
+    if (kind == 0) {   // wait for items, backing off (every poll hits the same counters)
+      for (int z = 0; z < idle; ++z) __builtin_amdgcn_s_sleep(127);
+      idle = imin(2 * idle + 1, 32);
       continue;
     }
+    idle = 0;
     if (kind == 1) {
       int32_t* it = p.items + (int64_t)arg * kItemInts;
       int ready = 1, unused = 0;
@@ -1462,10 +1522,6 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
       run_task(c, p, s, staged, lds, wv);
     } else {
       start_pair(c, p, arg, staged, lds, wv);
-    }
-    if (c.leader) {
-      __threadfence();
-      atomicSub(&q[4], 1);
     }
   }
 }

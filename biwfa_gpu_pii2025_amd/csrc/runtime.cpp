@@ -204,7 +204,7 @@ struct bw_aligner {
   bw_tuning_t tuning{};
   bw_progress_fn progress = nullptr;
   void* progress_user = nullptr;
-  int32_t* d_counters = nullptr;   // per tier: {queue, count, head, tail, busy, error}
+  int32_t* d_counters = nullptr;   // per tier: {queue, count, head, tail, in-flight task pairs, error}
   int32_t epoch_calls = 0;         // launches so far (ready-flag epochs)
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
   // last single-pair result
