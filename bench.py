@@ -81,12 +81,18 @@ def cpu_baseline(ds, res, min_core_s=12.0, threads=None):
 
     def view(m):
         return (ds.seq, ds.p_off[:m], ds.p_len[:m], ds.t_off[:m], ds.t_len[:m])
-    # calibrate on a small slice, then size the sample to ~min_core_s core-seconds
-    probe = min(n, 20000)
-    r = eng.align(view(probe), threads=threads)
+    # calibrate on a growing slice (8, 16, 32, ... pairs until it takes about half a
+    # second), then size the sample to ~min_core_s core-seconds
+    probe = min(n, 8)
+    while True:
+        r = eng.align(view(probe), threads=threads)
+        if r.elapsed >= 0.5 or probe >= n or probe >= 20000:
+            break
+        probe = min(n, 2 * probe)
     per_pair_core_s = max(r.elapsed * threads / probe, 1e-8)
     m = int(min(n, max(probe, min_core_s / per_pair_core_s)))
-    r = eng.align(view(m), threads=threads)
+    if m > probe:
+        r = eng.align(view(m), threads=threads)
     wall, reps = r.elapsed, 1
     while wall * threads < min_core_s:
         wall += eng.align(view(m), threads=threads).elapsed
@@ -97,7 +103,7 @@ def cpu_baseline(ds, res, min_core_s=12.0, threads=None):
     ref_off, ref_runs = r.compact()
     c_bad = _per_pair_mismatches(res.cigar_offset, res.cigar_runs, ref_off, ref_runs, m)
     # algorithmic work counters (cells, 64-bit words) from the restatement
-    cnt_m = min(m, 100000)
+    cnt_m = min(m, 100000, max(8, int(4.0 / per_pair_core_s)))   # a few core-seconds of counting
     oc = Oracle().align(view(cnt_m), threads=threads)
     o_off, o_runs = oc.compact()
     o_bad = _per_pair_mismatches(o_off, o_runs, ref_off, ref_runs, cnt_m) + \

@@ -338,8 +338,7 @@ static int base_score_bound(const Penalties& q) {
 // off as soon as a wavefront spans several chunks).
 static int team_waves(const bw_aligner_t* a, int wcap) {
   (void)a;
-  if (wcap <= 256) return 4;
-  if (wcap <= 1024) return 8;
+  if (wcap <= 1024) return 1;
   return 16;
 }
 

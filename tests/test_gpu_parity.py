@@ -216,3 +216,56 @@ def test_max_steps_status(aligners, oracle):
     ref = oracle.align(pairs, (0, 4, 6, 2), max_steps=60)
     res = aligners((0, 4, 6, 2), max_steps=60).align_batch(bw.pack_pairs(pairs))
     assert list(res.status) == list(ref.status)
+
+
+def test_tuning_changes_speed_not_results(aligners):
+    """Every team size and LDS ring width gives bit-identical results; the tuning
+    round-trips through the ABI and bad values are rejected."""
+    _say("tunings")
+    ds = bw.generate_dataset(bw.CONFIGS["C4"], num_pairs=120)
+    packed = ds.pack()
+    ref = None
+    for nw, lw in ((1, 16), (2, 32), (4, 64), (8, 32), (16, 128)):
+        a = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
+        t = {"lds_ring_width": lw, "lds_seq_cap": 0, "team_waves": [nw] * bw.MAX_TIERS,
+             "queue_chunk": [3] * bw.MAX_TIERS}
+        a.tuning = t
+        assert a.tuning["team_waves"] == [nw] * bw.MAX_TIERS and a.tuning["lds_ring_width"] == lw
+        res = a.align_batch(packed)
+        assert set(int(x) for x in res.status) == {0}
+        if ref is None:
+            ref = res
+        else:
+            assert np.array_equal(res.score, ref.score)
+            assert np.array_equal(res.cigar_offset, ref.cigar_offset)
+            assert np.array_equal(res.cigar_runs, ref.cigar_runs)
+        a.close()
+    a = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
+    with pytest.raises(bw.BiwfaError):
+        a.tuning = {"team_waves": [3]}
+    with pytest.raises(bw.BiwfaError):
+        a.tuning = {"lds_ring_width": 48}
+    a.close()
+
+
+def test_autotune_reports_and_keeps_results(aligners):
+    """bw_autotune times candidates (progress lines), keeps a valid tuning, and the
+    tuned aligner still aligns bit-exactly; per-tier pairs add up to the batch
+    plus the re-runs."""
+    _say("autotune")
+    ds = bw.generate_dataset(bw.CONFIGS["C4"], num_pairs=150)
+    packed = ds.pack()
+    a = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
+    lines = []
+    t = a.autotune(packed, repeats=1, progress=lines.append)
+    assert len(lines) >= 5 and all(line.startswith("autotune") for line in lines)
+    assert all(v in (1, 2, 4, 8, 16) for v in t["team_waves"][:6])
+    db = a.upload(packed)
+    db.align()
+    res = db.download()
+    pairs = a.tier_pairs()
+    assert sum(pairs) >= len(packed) and pairs[0] + sum(pairs[1:]) == sum(pairs)
+    ref = aligners((0, 4, 6, 2)).align_batch(packed)
+    assert np.array_equal(res.score, ref.score) and np.array_equal(res.cigar_runs, ref.cigar_runs)
+    db.free()
+    a.close()
