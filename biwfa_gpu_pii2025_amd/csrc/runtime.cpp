@@ -334,12 +334,12 @@ static int base_score_bound(const Penalties& q) {
 
 // Waves per aligner of the global tier with ring width `wcap` (the built-in
 // rule, from bw_autotune runs on the mixed-length, long-pair and 1 kbp batches:
-// a step is bound by the latency of its HBM round trips, so wider teams pay
-// off as soon as a wavefront spans several chunks).
+// a step is bound by the latency of its HBM round trips, and with enough pairs
+// to fill the GPU several small teams per CU hide it better than one wide team).
 static int team_waves(const bw_aligner_t* a, int wcap) {
   (void)a;
   if (wcap <= 1024) return 1;
-  return 16;
+  return 4;
 }
 
 static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b) {
@@ -823,11 +823,11 @@ static void report(const bw_aligner_t* a, const std::string& msg) {
   if (a->progress) a->progress(msg.c_str(), a->progress_user);
 }
 
-// Smallest team that keeps a wave at no more than 64 chunks of 64 diagonals per
-// step on a ring of width wcap (fewer lanes only make such a tier crawl).
+// Smallest team that keeps a wave at no more than 256 chunks of 64 diagonals
+// per step on a ring of width wcap (fewer lanes only make such a tier crawl).
 static int min_team_for(int wcap) {
   int nw = 1;
-  while (nw < 16 && wcap > 64 * 64 * nw) nw *= 2;
+  while (nw < 16 && wcap > 256 * 64 * nw) nw *= 2;
   return nw;
 }
 
