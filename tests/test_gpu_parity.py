@@ -269,3 +269,54 @@ def test_autotune_reports_and_keeps_results(aligners):
     assert np.array_equal(res.score, ref.score) and np.array_equal(res.cigar_runs, ref.cigar_runs)
     db.free()
     a.close()
+
+
+def _mutated_pairs(rng, n, lo, hi, rate):
+    pairs = []
+    for _ in range(n):
+        m = int(rng.integers(lo, hi))
+        p = "".join(rng.choice(list("ACGT"), m))
+        t = []
+        for ch in p:
+            r = rng.random()
+            if r < rate / 3:
+                t.append(str(rng.choice([c for c in "ACGT" if c != ch])))
+            elif r < 2 * rate / 3:
+                t.append(ch + "".join(rng.choice(list("ACGT"), int(rng.integers(1, 6)))))
+            elif r < rate:
+                continue
+            else:
+                t.append(ch)
+        pairs.append((p, "".join(t)))
+    return pairs
+
+
+@pytest.mark.parametrize("tag", list(PENALTY_SETS))
+def test_global_tiers_all_penalties(aligners, oracle, tag):
+    """Pairs too long for the LDS tier (1-6 kbp, so both the depth-first and the
+    shared-task paths of the global tiers run) under every utest penalty set,
+    including odd penalties (no null-score shortcut) and match < 0."""
+    pen = PENALTY_SETS[tag]
+    _say("global tiers penalties=%s" % (pen,))
+    rng = np.random.default_rng(17)
+    pairs = _mutated_pairs(rng, 24, 1000, 6000, 0.06)
+    ref = oracle.align(pairs, pen, threads=16)
+    res = aligners(pen).align_batch(bw.pack_pairs(pairs))
+    _compare(res, ref.alg_score, [ref.cigar(i) for i in range(len(pairs))], "global%s" % tag)
+    assert np.array_equal(res.wf_score, ref.wf_score)
+
+
+def test_global_tiers_step_cap_and_score_only(aligners, oracle):
+    """Step caps and score-only mode on pairs that run in the global tiers."""
+    _say("global tiers caps")
+    rng = np.random.default_rng(23)
+    pairs = _mutated_pairs(rng, 16, 2000, 7000, 0.08)
+    for cap in (300, 2000):
+        ref = oracle.align(pairs, (0, 4, 6, 2), max_steps=cap, threads=16)
+        res = aligners((0, 4, 6, 2), max_steps=cap).align_batch(bw.pack_pairs(pairs))
+        assert list(res.status) == list(ref.status), cap
+    ref = oracle.align(pairs, (0, 4, 6, 2), score_only=True, threads=16)
+    res = aligners((0, 4, 6, 2), score_only=True).align_batch(bw.pack_pairs(pairs))
+    assert set(int(s) for s in res.status) == {0}
+    assert np.array_equal(res.wf_score, ref.wf_score)
+    assert np.array_equal(res.score, ref.alg_score)
