@@ -692,6 +692,111 @@ DI void accept_hit(const Ctx<K>& c, const int32_t* buf, int j, bool forward, int
   bp.comp = comp;
 }
 
+// Lane-parallel form of the overlap test below for a single-wave aligner with
+// its rings in LDS.  The sequential order visits i = 0..scope-1 and at each
+// score s1-i the D, I and M components, taking a hit only when it strictly
+// improves the breakpoint; the breakpoint it ends with is therefore the hit
+// with the smallest score, ties going to the earliest visit, and within one
+// component the smallest k.  Here lane p = 3i + {0: D, 1: I, 2: M} sets up the
+// candidate of one visit (header gather, presence, diagonal intersection), the
+// candidates that can improve are compacted into a table, every (candidate, k)
+// is tested in fixed-width lane slots, and the key (score, p, k) is minimised
+// over the wave.  Requires 3*scope <= 64, wcap <= 64, ring offsets < 4096,
+// 2*3*scope <= rcap (the table borrows the backtrace run buffer, idle during a
+// breakpoint search) and s0 + s1 < 2^18 (key width); the caller checks.
+DI int lane_div3(int p) { return (p * 43) >> 7; }   // p / 3 for 0 <= p < 128
+// v0, v1 or v2 by a per-lane index 0..2, as masking arithmetic (see sel3)
+DI int pick3(int v0, int v1, int v2, int w) {
+  return (v0 & -(int)(w == 0)) | (v1 & -(int)(w == 1)) | (v2 & -(int)(w == 2));
+}
+
+template <class K>
+DI void overlap_lanes(Ctx<K>& c, const Dir& a1, int s0, int s1, int slot1, bool forward,
+                      const Wf& m0, const Wf& i0, const Wf& d0, int pi0, int pd0, int tl, int pl,
+                      BP& bp) {
+  const int dk = tl - pl;
+  const int p = c.lane;
+  const int i = lane_div3(p);
+  const int cc = p - 3 * i;                          // 0: D, 1: I, 2: M
+  const int si = s1 - i;
+  const int sc = s0 + si - (cc == 2 ? 0 : c.o);
+  int sl = slot1 - i;
+  sl += sl < 0 ? c.scope : 0;
+  const int comp = cc == 0 ? CD : cc == 1 ? CI : CM;
+  const uint64_t pres = sel3u(c.rpres, comp);
+  const int j = a1.ring * c.scope + sl;              // header lane of a1's slot
+  const int jb = (j & (kWave - 1)) << 2;
+  const int lo_m = __builtin_amdgcn_ds_bpermute(jb, c.rlo[CM]);
+  const int lo_i = __builtin_amdgcn_ds_bpermute(jb, c.rlo[CI]);
+  const int lo_d = __builtin_amdgcn_ds_bpermute(jb, c.rlo[CD]);
+  const int hi_m = __builtin_amdgcn_ds_bpermute(jb, c.rhi[CM]);
+  const int hi_i = __builtin_amdgcn_ds_bpermute(jb, c.rhi[CI]);
+  const int hi_d = __builtin_amdgcn_ds_bpermute(jb, c.rhi[CD]);
+  const int lo1 = pick3(lo_d, lo_i, lo_m, cc);
+  const int hi1 = pick3(hi_d, hi_i, hi_m, cc);
+  const int lo0 = pick3(d0.lo, i0.lo, m0.lo, cc);
+  const int hi0 = pick3(d0.hi, i0.hi, m0.hi, cc);
+  const int b0 = pick3(d0.base, i0.base, m0.base, cc);
+  const int b1 = c.L.data + (j * 3 + comp) * c.L.wcap;
+  const bool p0 = pick3(pd0, pi0, 1, cc) != 0;
+  const int klo = imax(lo0, dk - hi1), khi = imin(hi0, dk - lo1);
+  const bool ok = i < c.scope && si >= 0 && sc < bp.score && p0 &&
+                  ((pres >> (a1.ring * 32 + sl)) & 1ull) && klo <= khi;
+  const uint64_t vm = __ballot(ok);
+  if (!vm) return;
+  const int nr = __builtin_popcountll(vm);
+  int32_t* const tab = c.pv + c.L.runs_tmp;
+  if (ok) {
+    const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(vm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)vm, 0));
+    tab[2 * r] = (klo & 0xffff) | (khi << 16);
+    tab[2 * r + 1] = b0 | (b1 << 12) | (p << 24);
+  }
+  wave_sync<K::kLds>();
+  const int wm = c.L.wcap - 1;
+  const int sh = __builtin_ctz(c.L.wcap);
+  const int off = c.lane & wm;
+  int best = INT_MAX;
+  for (int r0 = 0; r0 < nr; r0 += kWave >> sh) {
+    const int r = r0 + (c.lane >> sh);
+    if (r < nr) {
+      const int e0 = tab[2 * r], e1 = tab[2 * r + 1];
+      const int k0 = (int)(int16_t)(e0 & 0xffff) + off;
+      if (k0 <= (e0 >> 16)) {
+        const int k1 = dk - k0;
+        const int pp = (e1 >> 24) & 63;
+        const int o0 = c.ws[(e1 & 0xfff) + (k0 & wm)];
+        const int o1 = c.ws[((e1 >> 12) & 0xfff) + (k1 & wm)];
+        const int ii = lane_div3(pp);
+        const bool indel = pp - 3 * ii != 2;
+        bool hit = o0 + o1 >= tl;
+        if (hit && indel) hit = forward ? !(o0 - k0 > pl || o0 > tl) : !(o1 - k1 > pl || o1 > tl);
+        if (hit) {
+          const int scc = s0 + s1 - ii - (indel ? c.o : 0);
+          best = imin(best, (scc << 12) | (pp << 6) | off);
+        }
+      }
+    }
+  }
+  const int key = INT_MAX - wave_max_nonneg(INT_MAX - best);
+  // the table is read before anything else writes the run buffer
+  wave_sync<K::kLds>();
+  if (key == INT_MAX) return;
+  const int pp = (key >> 6) & 63;
+  const int ii = lane_div3(pp);
+  const int ccw = pp - 3 * ii;
+  const int s1w = s1 - ii;
+  const int kk0 = rdl(klo, pp) + (key & 63), kk1 = dk - kk0;
+  const int a0v = uni(c.ws[rdl(b0, pp) + (kk0 & wm)]);
+  const int a1v = uni(c.ws[rdl(b1, pp) + (kk1 & wm)]);
+  if (forward) {
+    bp.sf = s0; bp.sr = s1w; bp.kf = kk0; bp.kr = kk1; bp.of = a0v; bp.orr = a1v;
+  } else {
+    bp.sf = s1w; bp.sr = s0; bp.kf = kk1; bp.kr = kk0; bp.of = a1v; bp.orr = a0v;
+  }
+  bp.score = key >> 12;
+  bp.comp = ccw == 0 ? CD : ccw == 1 ? CI : CM;
+}
+
 template <class K>
 DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int s1,
                 int slot1, bool forward, BP& bp) {
@@ -703,6 +808,13 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
   const int tl = a0.tlen, pl = a0.plen;
   // ring slots of a1 holding any component (null scores hold none)
   const uint64_t live1 = ((c.rpres[0] | c.rpres[1] | c.rpres[2]) >> (32 * a1.ring)) & 0xffffffffull;
+  if constexpr (K::kLds) {
+    if (3 * c.scope <= kWave && c.L.wcap <= kWave && 2 * 3 * c.scope * c.L.wcap <= 4096 &&
+        6 * c.scope <= c.L.rcap && s0 + s1 < (1 << 18)) {
+      overlap_lanes(c, a1, s0, s1, slot1, forward, m0, i0, d0, pi0, pd0, tl, pl, bp);
+      return;
+    }
+  }
   if constexpr (K::kNW == 1) {
     for (int i = 0; i < c.scope; ++i) {
       const int si = s1 - i;
