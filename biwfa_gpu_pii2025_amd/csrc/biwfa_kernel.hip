@@ -102,6 +102,13 @@ DI Wf wf_null() { return Wf{1, -1, 0, 0}; }
 DI int rd(const int32_t* ws, const Wf& w, int k) {
   return (k >= w.lo && k <= w.hi) ? ws[w.base + (k & w.mask)] : kOffsetNull;
 }
+// rd without a branch: every lane loads, at k clamped into [lo, hi] (for the
+// null wavefront lo=1, hi=-1, mask 0: index base+0), and lanes outside take NULL.
+DI int rd_clamped(const int32_t* ws, const Wf& w, int k) {
+  const int kk = imin(imax(k, w.lo), w.hi);
+  const int v = ws[w.base + (kk & w.mask)];
+  return (k >= w.lo && k <= w.hi) ? v : kOffsetNull;
+}
 
 // One aligner's view of a sub-problem.
 struct Dir {
@@ -473,8 +480,38 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   const int k_end = d.tlen - d.plen;
   const int ce = d.ce;
   int fm = INT_MAX, lm = INT_MIN, fi = INT_MAX, li = INT_MIN, fd = INT_MAX, ld = INT_MIN;
-  int lmax = 0, endv = INT_MIN;
+  int lmax = 0, endv = INT_MIN, endu = INT_MIN;
   int32_t* const ws = c.ws;
+  if (K::kLds && width <= kWave) {
+    // One chunk holds the whole wavefront (always, for ring widths up to 64):
+    // no loop, branch-free input reads, trims straight from the ballots.
+    const int k = lo + c.lane;
+    const bool act = k <= hi;
+    const int ins1 = imax(rd_clamped(ws, mo, k - 1), rd_clamped(ws, ie, k - 1)) + 1;
+    const int del1 = imax(rd_clamped(ws, mo, k + 1), rd_clamped(ws, de, k + 1));
+    int mx = imax(del1, imax(rd_clamped(ws, mm, k) + 1, ins1));
+    const bool live = act && !((uint32_t)mx > tl || (uint32_t)(mx - k) > pl);
+    int ak = 0;
+    if (live) {                                   // extension along the diagonal
+      mx += lcp(c, d, mx - k, mx);
+      ak = 2 * mx - k;
+    }
+    mx = live ? mx : kOffsetNull;
+    if (act) {
+      ws[bm + (k & mask)] = mx;
+      if (has_i) ws[bi + (k & mask)] = ins1;
+      if (has_d) ws[bd + (k & mask)] = del1;
+    }
+    const uint64_t b_m = __ballot(live);
+    const uint64_t b_i = __ballot(act && (uint32_t)ins1 <= tl && (uint32_t)(ins1 - k) <= pl);
+    const uint64_t b_d = __ballot(act && (uint32_t)del1 <= tl && (uint32_t)(del1 - k) <= pl);
+    if (b_m) { fm = lo + __builtin_ctzll(b_m); lm = lo + 63 - __builtin_clzll(b_m); }
+    if (b_i) { fi = lo + __builtin_ctzll(b_i); li = lo + 63 - __builtin_clzll(b_i); }
+    if (b_d) { fd = lo + __builtin_ctzll(b_d); ld = lo + 63 - __builtin_clzll(b_d); }
+    const int ev = (ce == CM) ? mx : (ce == CI) ? ins1 : del1;
+    if (k_end >= lo && k_end <= hi) endu = rdl(ev, k_end - lo);
+    lmax = wave_max_nonneg(ak);
+  } else {
   // A wave takes kG of its chunks at a time and issues all their loads before
   // any store: the stores of one chunk could alias the next chunk's inputs as
   // far as the compiler knows, so without the grouping every chunk would wait
@@ -522,8 +559,9 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     }
   }
   // offset at k_end: held by lane (k_end - lo) % 64 of the wave owning its chunk
-  int endu = (k_end >= lo && k_end <= hi) ? rdl(endv, (k_end - lo) & (kWave - 1)) : INT_MIN;
+  if (k_end >= lo && k_end <= hi) endu = rdl(endv, (k_end - lo) & (kWave - 1));
   lmax = wave_max_nonneg(lmax);
+  }
   if constexpr (K::kNW > 1) {   // combine the team's waves
     int nfm = -fm, nfi = -fi, nfd = -fd;
     team_max8(c, nfm, lm, nfi, li, nfd, ld, lmax, endu);
