@@ -106,7 +106,7 @@ DI int rd(const int32_t* ws, const Wf& w, int k) {
 // null wavefront lo=1, hi=-1, mask 0: index base+0), and lanes outside take NULL.
 DI int rd_clamped(const int32_t* ws, const Wf& w, int k) {
   const int kk = imin(imax(k, w.lo), w.hi);
-  const int v = ws[w.base + (kk & w.mask)];
+  const int v = ws[(uint32_t)(w.base + (kk & w.mask))];
   return (k >= w.lo && k <= w.hi) ? v : kOffsetNull;
 }
 
@@ -343,34 +343,39 @@ DI void set_hdr(Ctx<K>& c, const Dir& d, int score, int slot, int comp, int lo, 
 // ---------------------------------------------------------------------------
 // sequences: 2-bit packed, base j at bits 2*(j%32) of word j/32
 
-DI uint64_t window(const uint64_t* s, int pos) {       // bases [pos, pos+32)
-  const int wi = pos >> 5;
-  const int sh = (pos & 31) << 1;
-  const uint64_t lo = s[wi], hi = s[wi + 1];
-  return (lo >> sh) | ((hi << 1) << (63 - sh));
+// Bases [32w + sh, 32w + sh + 32) from the two words lo = word w, hi = word w+1.
+DI uint64_t window2(uint64_t lo, uint64_t hi, int sh) {
+  const int b = sh << 1;
+  return (lo >> b) | ((hi << 1) << (63 - b));
 }
 
 // Longest common extension from (v,h) of the aligner's sub-problem, 32 bases
-// per 64-bit compare, clamped to the sub-problem's end.
+// per 64-bit compare, clamped to the sub-problem's end.  The four words of an
+// iteration are loaded before either window is formed, so the two sequence
+// reads overlap.
 template <class K>
 DI int lcp(const Ctx<K>& c, const Dir& d, int v, int h) {
   const int lim = imin(d.plen - v, d.tlen - h);
+  // forward: windows start at P[pb+v+n], T[tb+h+n]; reverse: windows end at
+  // P[pe-1-v-n], T[te-1-h-n] and the first mismatch is the highest differing base
+  const bool rev = d.rev != 0;
+  int pp = rev ? d.pb + d.plen - 32 - v : d.pb + v;
+  int tp = rev ? d.tb + d.tlen - 32 - h : d.tb + h;
+  const int step = rev ? -32 : 32;
   int n = 0;
-  if (!d.rev) {
-    const int pp = d.pb + v, tp = d.tb + h;
-    while (n < lim) {
-      const uint64_t x = window(c.P, pp + n) ^ window(c.T, tp + n);
-      if (x) { n += __builtin_ctzll(x) >> 1; break; }
-      n += 32;
+  for (;;) {
+    const int wp = pp >> 5, wt = tp >> 5;
+    const uint64_t p0 = c.P[wp], p1 = c.P[wp + 1];
+    const uint64_t t0 = c.T[wt], t1 = c.T[wt + 1];
+    const uint64_t x = window2(p0, p1, pp & 31) ^ window2(t0, t1, tp & 31);
+    if (x) {
+      n += (rev ? __builtin_clzll(x) : __builtin_ctzll(x)) >> 1;
+      break;
     }
-  } else {
-    // walking down from P[pe-1-v], T[te-1-h]: windows end at the current base
-    const int pp = d.pb + d.plen - 1 - v, tp = d.tb + d.tlen - 1 - h;
-    while (n < lim) {
-      const uint64_t x = window(c.P, pp - n - 31) ^ window(c.T, tp - n - 31);
-      if (x) { n += __builtin_clzll(x) >> 1; break; }
-      n += 32;
-    }
+    n += 32;
+    if (n >= lim) break;
+    pp += step;
+    tp += step;
   }
   return imin(n, lim);
 }
