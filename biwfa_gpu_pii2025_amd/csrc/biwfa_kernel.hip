@@ -100,7 +100,7 @@ struct Wf {
 };
 DI Wf wf_null() { return Wf{1, -1, 0, 0}; }
 DI int rd(const int32_t* ws, const Wf& w, int k) {
-  return (k >= w.lo && k <= w.hi) ? ws[w.base + (k & w.mask)] : kOffsetNull;
+  return (k >= w.lo && k <= w.hi) ? ws[(uint32_t)(w.base + (k & w.mask))] : kOffsetNull;
 }
 // rd without a branch: every lane loads, at k clamped into [lo, hi] (for the
 // null wavefront lo=1, hi=-1, mask 0: index base+0), and lanes outside take NULL.
@@ -130,6 +130,7 @@ struct Ctx {
   int32_t* pv;    // this wave's private region (base headers, backtrace runs, stack)
   const uint64_t* P;
   const uint64_t* T;
+  int seq_lds;    // P and T point into LDS (staged), else into global memory
   int lane;
   int wid;        // wave index within the team (0 for a single-wave aligner)
   bool leader;    // lane 0 of wave 0: the one lane that writes results
@@ -353,8 +354,22 @@ DI uint64_t window2(uint64_t lo, uint64_t hi, int sh) {
 // per 64-bit compare, clamped to the sub-problem's end.  The four words of an
 // iteration are loaded before either window is formed, so the two sequence
 // reads overlap.
+typedef const __attribute__((address_space(3))) uint64_t* lds_words;
+typedef const __attribute__((address_space(1))) uint64_t* global_words;
+
+template <class Ptr>
+DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h);
+
+// Extension with the sequence loads in the address space the pair's words
+// live in (generic loads would wait on both the LDS and the memory counters).
 template <class K>
 DI int lcp(const Ctx<K>& c, const Dir& d, int v, int h) {
+  if (K::kLds || c.seq_lds) return lcp_words((lds_words)c.P, (lds_words)c.T, d, v, h);
+  return lcp_words((global_words)c.P, (global_words)c.T, d, v, h);
+}
+
+template <class Ptr>
+DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h) {
   const int lim = imin(d.plen - v, d.tlen - h);
   // forward: windows start at P[pb+v+n], T[tb+h+n]; reverse: windows end at
   // P[pe-1-v-n], T[te-1-h-n] and the first mismatch is the highest differing base
@@ -365,8 +380,8 @@ DI int lcp(const Ctx<K>& c, const Dir& d, int v, int h) {
   int n = 0;
   for (;;) {
     const int wp = pp >> 5, wt = tp >> 5;
-    const uint64_t p0 = c.P[wp], p1 = c.P[wp + 1];
-    const uint64_t t0 = c.T[wt], t1 = c.T[wt + 1];
+    const uint64_t p0 = P[wp], p1 = P[wp + 1];
+    const uint64_t t0 = T[wt], t1 = T[wt + 1];
     const uint64_t x = window2(p0, p1, pp & 31) ^ window2(t0, t1, tp & 31);
     if (x) {
       n += (rev ? __builtin_clzll(x) : __builtin_ctzll(x)) >> 1;
@@ -549,9 +564,9 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
         mx = kOffsetNull;
       }
       if (act) {
-        ws[bm + (k & mask)] = mx;
-        if (has_i) ws[bi + (k & mask)] = ins1[g];
-        if (has_d) ws[bd + (k & mask)] = del1[g];
+        ws[(uint32_t)(bm + (k & mask))] = mx;
+        if (has_i) ws[(uint32_t)(bi + (k & mask))] = ins1[g];
+        if (has_d) ws[(uint32_t)(bd + (k & mask))] = del1[g];
         if (k == k_end) endv = (ce == CM) ? mx : (ce == CI) ? ins1[g] : del1[g];
       }
       // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
@@ -1388,6 +1403,7 @@ DI void stage_pair(Ctx<K>& c, const KParams& p, int pair, int32_t* lds, int wv) 
   const uint64_t* gT = p.words + p.t_woff[pair];
   c.P = gP;
   c.T = gT;
+  c.seq_lds = 0;
   const int pw = (plen + 31) >> 5, tw = (tlen + 31) >> 5;
   const int sw = p.seq_lds_words;
   if (pw + 3 <= sw && tw + 3 <= sw) {
@@ -1400,6 +1416,7 @@ DI void stage_pair(Ctx<K>& c, const KParams& p, int pair, int32_t* lds, int wv) 
     team_sync<K>();
     c.P = sP + 1;
     c.T = sT + 1;
+    c.seq_lds = 1;
   }
 }
 
@@ -1750,6 +1767,7 @@ void biwfa_persistent(KParams p) {
         }
         c.P = sP + 1;
         c.T = sT + 1;
+        c.seq_lds = 1;
         wave_sync<K::kLds>();
       }
       int wf = INT_MIN, st = kStCapacity;
