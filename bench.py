@@ -10,7 +10,11 @@ x 150 bp, 2% edits, x=4 o=6 e=2, length-sorted schedule).
 
 Prints one JSON line.  Besides the contract fields it carries
   roofline      -- HBM roofline of the dominant kernel (algorithmic bytes / kernel time),
-  int_roofline  -- the integer-VALU roofline the path is actually bound by,
+  issue_roofline -- scalar-instruction issue rate of that kernel (the bound it runs
+                   against: one SALU instruction per CU per clock), from the kernel's
+                   per-pair SQ counts in profiles/ and the live kernel time,
+  int_roofline  -- the algorithmic integer work (cells and compares) against the
+                   int32 VALU peak,
   cpu_baseline  -- WFA2-lib BiWFA (built from the reference sources) with OpenMP
                    on this host's cores, on a bounded sample of the same workload,
   parity        -- GPU vs CPU reference on that sample (scores and CIGARs).
@@ -27,6 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0                  # MI355X HBM3E
+# instruction issue: one scalar (SALU) and one vector (VALU) wave instruction per CU
+# per clock, 256 CUs at 2.4 GHz
+ISSUE_PEAK_G = 256 * 2.4
 # int32 VALU peak: 256 CUs x 4 SIMD x 16 lanes per clock at 2.4 GHz
 INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 
@@ -55,6 +62,20 @@ def traffic_bytes(workload, n):
     if t.get("workload") != workload or int(t.get("pairs", -1)) != n:
         return None
     return int(t["traffic_bytes"])
+
+
+def sq_counts(workload, n):
+    """Per-pair SALU / VALU wave-instruction counts of the dominant kernel from the
+    committed rocprofv3 SQ counter pass, when it was taken on this same workload."""
+    path = os.path.join(ROOT, "profiles", "c2_sq.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or int(t.get("pairs", -1)) != n:
+        return None
+    return t
 
 
 def _per_pair_mismatches(off_a, runs_a, off_b, runs_b, m):
@@ -285,6 +306,17 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes,
         },
     }
+    sq = sq_counts(args.workload, n)
+    if sq is not None:
+        salu_g = sq["salu_per_pair"] * n / main_s / 1e9
+        valu_g = sq["valu_per_pair"] * n / main_s / 1e9
+        out["issue_roofline"] = {
+            "bound": "salu-issue", "achieved": round(salu_g, 1), "peak": ISSUE_PEAK_G,
+            "unit": "G instr/s", "frac": round(salu_g / ISSUE_PEAK_G, 4),
+            "valu_frac": round(valu_g / ISSUE_PEAK_G, 4),
+            "salu_per_pair": round(sq["salu_per_pair"], 1), "valu_per_pair": round(sq["valu_per_pair"], 1),
+            "counts_from": "profiles/biwfa_persistent_lds_C2_1M_sq_counters.md",
+        }
     if not args.no_cpu:
         t0 = time.time()
         cpu, parity, counters = cpu_baseline(ds, res)
