@@ -54,6 +54,12 @@ DI int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 DI int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 DI int imax(int a, int b) { return a >= b ? a : b; }
 DI int imin(int a, int b) { return a <= b ? a : b; }
+// v0, v1 or v2 by an index 0..2 that may differ between lanes, written as
+// masking arithmetic: a select of register-array elements would be folded into
+// a select of their addresses and pin the array to scratch.
+DI int pick3(int v0, int v1, int v2, int w) {
+  return (v0 & -(int)(w == 0)) | (v1 & -(int)(w == 1)) | (v2 & -(int)(w == 2));
+}
 
 // Orders this wave's workspace writes before later reads by other lanes.  LDS
 // operations of one wave execute in issue order, so the LDS kernel only needs
@@ -431,6 +437,78 @@ DI void base_inputs(const Ctx<K>& c, int s, Wf& mm, Wf& mo, Wf& ie, Wf& de) {
   mm = w[0]; mo = w[1]; ie = w[2]; de = w[3];
 }
 
+// The four inputs of a step -- M[s-x], M[s-o-e], I1[s-e], D1[s-e] -- and the
+// limits they give (wavefront_compute_limits_input, wavefront_compute.c:40-86),
+// for a single-wave LDS aligner, lane-parallel: lane q < 4 fetches input q's
+// header from the header lanes (ds_bpermute), tests its presence, normalises an
+// absent or empty input to the null wavefront (lo=1, hi=-1, which still takes
+// part in the limits as in WFA2), and the limits are reduced over lanes 0..3.
+// Same results as four get_in calls, with the per-input control done in vector
+// lanes instead of one scalar chain per input.  Returns false when all four
+// inputs are null.  Ring inputs use mask wcap-1, base-case inputs mask -1; a
+// null input's base is chosen so that a clamped read stays in the workspace.
+template <class K>
+DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf& mo, Wf& ie, Wf& de,
+                      int& lo, int& hi, bool& has_i, bool& has_d) {
+  const int q = c.lane;
+  const int delta = pick3(c.x, c.o + c.e, c.e, imin(q, 2));   // q = 3 shares s-e with q = 2
+  const int cq = q <= 1 ? CM : q == 2 ? CI : CD;
+  const int sq = s - delta;
+  const bool ring = d.ring >= 0;
+  int hl, pbit, lq, hq, kq;
+  uint64_t pm;
+  if (ring) {
+    int sl = slot - delta;
+    sl += sl < 0 ? c.scope : 0;
+    hl = d.ring * c.scope + sl;
+    pbit = d.ring * 32 + sl;
+    const int hb = (hl & (kWave - 1)) << 2;
+    lq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rlo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rlo[CI]),
+               __builtin_amdgcn_ds_bpermute(hb, c.rlo[CD]), cq);
+    hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
+               __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
+    kq = c.L.data + (hl * 3 + cq) * c.L.wcap;
+    pm = sel3u(c.rpres, cq);
+  } else {
+    hl = sq;
+    pbit = sq & (kWave - 1);
+    const int hb = (hl & (kWave - 1)) << 2;
+    lq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.blo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.blo[CI]),
+               __builtin_amdgcn_ds_bpermute(hb, c.blo[CD]), cq);
+    hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bhi[CI]),
+               __builtin_amdgcn_ds_bpermute(hb, c.bhi[CD]), cq);
+    kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
+               __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
+    pm = sel3u(c.bpres, cq);
+  }
+  const bool present = q < 4 && sq >= 0 && ((pm >> pbit) & 1ull);
+  const bool nonempty = present && lq <= hq;
+  if (!nonempty) {
+    lq = 1;
+    hq = -1;
+    kq = ring ? 0 : 1;
+  }
+  // limits: M[s-x] [lo, hi], M[s-o-e] [lo-1, hi+1], I1[s-e] [lo+1, hi+1], D1[s-e] [lo-1, hi-1]
+  int lc = lq + (q == 0 ? 0 : q == 2 ? 1 : -1);
+  int hc = hq + (q == 0 ? 0 : q == 3 ? -1 : 1);
+  if (q >= 4) { lc = INT_MAX; hc = INT_MIN; }
+  lc = imin(lc, __builtin_amdgcn_update_dpp(INT_MAX, lc, 0x111, 0xf, 0xf, false));  // row_shr:1
+  hc = imax(hc, __builtin_amdgcn_update_dpp(INT_MIN, hc, 0x111, 0xf, 0xf, false));
+  lc = imin(lc, __builtin_amdgcn_update_dpp(INT_MAX, lc, 0x112, 0xf, 0xf, false));  // row_shr:2
+  hc = imax(hc, __builtin_amdgcn_update_dpp(INT_MIN, hc, 0x112, 0xf, 0xf, false));
+  const uint32_t ne = (uint32_t)__ballot(nonempty) & 0xfu;
+  const int mk = ring ? c.L.wcap - 1 : -1;
+  mm = Wf{rdl(lq, 0), rdl(hq, 0), rdl(kq, 0), mk};
+  mo = Wf{rdl(lq, 1), rdl(hq, 1), rdl(kq, 1), mk};
+  ie = Wf{rdl(lq, 2), rdl(hq, 2), rdl(kq, 2), mk};
+  de = Wf{rdl(lq, 3), rdl(hq, 3), rdl(kq, 3), mk};
+  lo = rdl(lc, 3);
+  hi = rdl(hc, 3);
+  has_i = (ne & 0x6u) != 0;
+  has_d = (ne & 0xau) != 0;
+  return ne != 0;
+}
+
 // One WFA step at score s: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
 // followed by wavefront_extend_end2end(_max) (wavefront_extend.c:86-211) and the
 // termination test (wavefront_termination.c:37-114), fused: the lane that
@@ -450,16 +528,32 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   // With penalties that share a power-of-two factor g, only multiples of g are
   // reachable scores: the other steps are null without looking at the inputs.
   Wf mm, mo, ie, de;
-  if (!null) {
-    if (!K::kLds && d.ring < 0) {
-      base_inputs(c, s, mm, mo, ie, de);
-    } else {
-      mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
-      mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
-      ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
-      de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
+  int lo = 0, hi = 0;
+  bool has_i = false, has_d = false;
+  if constexpr (K::kLds) {
+    if (!null) null = !gather_inputs(c, d, s, slot, mm, mo, ie, de, lo, hi, has_i, has_d);
+  } else {
+    if (!null) {
+      if (d.ring < 0) {
+        base_inputs(c, s, mm, mo, ie, de);
+      } else {
+        mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
+        mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
+        ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
+        de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
+      }
+      null = mm.lo > mm.hi && mo.lo > mo.hi && ie.lo > ie.hi && de.lo > de.hi;
     }
-    null = mm.lo > mm.hi && mo.lo > mo.hi && ie.lo > ie.hi && de.lo > de.hi;
+    if (!null) {
+      const bool nmo = mo.lo > mo.hi, nie = ie.lo > ie.hi, nde = de.lo > de.hi;
+      // wavefront_compute_limits_input (wavefront_compute.c:40-86)
+      lo = mm.lo; hi = mm.hi;
+      lo = imin(lo, mo.lo - 1); hi = imax(hi, mo.hi + 1);
+      lo = imin(lo, ie.lo + 1); hi = imax(hi, ie.hi + 1);
+      lo = imin(lo, de.lo - 1); hi = imax(hi, de.hi - 1);
+      has_i = !nmo || !nie;
+      has_d = !nmo || !nde;
+    }
   }
   if (null) {                                   // null step (wavefront_compute_affine.c:245-251)
     null_step(c, d, s, slot);
@@ -470,14 +564,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     }
     return 0;
   }
-  const bool nmo = mo.lo > mo.hi, nie = ie.lo > ie.hi, nde = de.lo > de.hi;
   d.null_steps = 0;
-  // wavefront_compute_limits_input (wavefront_compute.c:40-86)
-  int lo = mm.lo, hi = mm.hi;
-  lo = imin(lo, mo.lo - 1); hi = imax(hi, mo.hi + 1);
-  lo = imin(lo, ie.lo + 1); hi = imax(hi, ie.hi + 1);
-  lo = imin(lo, de.lo - 1); hi = imax(hi, de.hi - 1);
-  const bool has_i = !nmo || !nie, has_d = !nmo || !nde;
   const int width = hi - lo + 1;
   int bm, bi, bd, mask;
   if (d.ring >= 0) {
@@ -763,10 +850,6 @@ DI void accept_hit(const Ctx<K>& c, const int32_t* buf, int j, bool forward, int
 // 2*3*scope <= rcap (the table borrows the backtrace run buffer, idle during a
 // breakpoint search) and s0 + s1 < 2^18 (key width); the caller checks.
 DI int lane_div3(int p) { return (p * 43) >> 7; }   // p / 3 for 0 <= p < 128
-// v0, v1 or v2 by a per-lane index 0..2, as masking arithmetic (see sel3)
-DI int pick3(int v0, int v1, int v2, int w) {
-  return (v0 & -(int)(w == 0)) | (v1 & -(int)(w == 1)) | (v2 & -(int)(w == 2));
-}
 
 template <class K>
 DI void overlap_lanes(Ctx<K>& c, const Dir& a1, int s0, int s1, int slot1, bool forward,
@@ -1727,6 +1810,10 @@ void biwfa_persistent(KParams p) {
   c.gmask = p.gmask;
   const int count = uni(*(volatile const int32_t*)p.count);
   if constexpr (!K::kLds) {
+    // The list is final at launch (the tier below has finished); with no pairs
+    // there are no shared tasks either, so an empty tier ends at once instead
+    // of every aligner polling the queue counters.
+    if (count == 0) return;
     task_loop(c, p, count, lds, wv);
   } else {
   // LDS tier: one wave per pair, depth-first recursion on the wave alone
