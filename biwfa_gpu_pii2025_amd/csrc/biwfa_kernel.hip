@@ -556,7 +556,14 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     }
   }
   if (null) {                                   // null step (wavefront_compute_affine.c:245-251)
-    null_step(c, d, s, slot);
+    if (K::kLds && (s & c.gmask) != 0) {
+      // A score that is not a multiple of g is never read in a single-wave
+      // aligner: step inputs are s minus multiples of g, the overlap test and
+      // the backtrace skip such scores.  Its headers are left as they are.
+      d.null_steps++;
+    } else {
+      null_step(c, d, s, slot);
+    }
     if (d.null_steps > c.scope) {               // nothing left to extend (wavefront_extend.c)
       d.status = kStEndUnreachable;
       d.status_score = s;
@@ -852,9 +859,8 @@ DI void accept_hit(const Ctx<K>& c, const int32_t* buf, int j, bool forward, int
 DI int lane_div3(int p) { return (p * 43) >> 7; }   // p / 3 for 0 <= p < 128
 
 template <class K>
-DI void overlap_lanes(Ctx<K>& c, const Dir& a1, int s0, int s1, int slot1, bool forward,
-                      const Wf& m0, const Wf& i0, const Wf& d0, int pi0, int pd0, int tl, int pl,
-                      BP& bp) {
+DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0, int s1, int slot1,
+                      bool forward, int tl, int pl, BP& bp) {
   const int dk = tl - pl;
   const int p = c.lane;
   const int i = lane_div3(p);
@@ -875,13 +881,15 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a1, int s0, int s1, int slot1, bool 
   const int hi_d = __builtin_amdgcn_ds_bpermute(jb, c.rhi[CD]);
   const int lo1 = pick3(lo_d, lo_i, lo_m, cc);
   const int hi1 = pick3(hi_d, hi_i, hi_m, cc);
-  const int lo0 = pick3(d0.lo, i0.lo, m0.lo, cc);
-  const int hi0 = pick3(d0.hi, i0.hi, m0.hi, cc);
-  const int b0 = pick3(d0.base, i0.base, m0.base, cc);
+  // a0's component of this lane's candidate, at score s0 (header lane j0)
+  const int j0 = a0.ring * c.scope + slot0;
+  const int lo0 = pick3(rdl(c.rlo[CD], j0), rdl(c.rlo[CI], j0), rdl(c.rlo[CM], j0), cc);
+  const int hi0 = pick3(rdl(c.rhi[CD], j0), rdl(c.rhi[CI], j0), rdl(c.rhi[CM], j0), cc);
+  const int b0 = c.L.data + (j0 * 3 + comp) * c.L.wcap;
   const int b1 = c.L.data + (j * 3 + comp) * c.L.wcap;
-  const bool p0 = pick3(pd0, pi0, 1, cc) != 0;
+  const bool p0 = (pres >> (a0.ring * 32 + slot0)) & 1ull;
   const int klo = imax(lo0, dk - hi1), khi = imin(hi0, dk - lo1);
-  const bool ok = i < c.scope && si >= 0 && sc < bp.score && p0 &&
+  const bool ok = i < c.scope && si >= 0 && (si & c.gmask) == 0 && sc < bp.score && p0 &&
                   ((pres >> (a1.ring * 32 + sl)) & 1ull) && klo <= khi;
   const uint64_t vm = __ballot(ok);
   if (!vm) return;
@@ -941,26 +949,28 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a1, int s0, int s1, int slot1, bool 
 template <class K>
 DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int s1,
                 int slot1, bool forward, BP& bp) {
+  if (s0 & c.gmask) return;     // a score that is not a multiple of g holds nothing
+  const int tl = a0.tlen, pl = a0.plen;
+  if constexpr (K::kLds) {
+    if (3 * c.scope <= kWave && c.L.wcap <= kWave && 2 * 3 * c.scope * c.L.wcap <= 4096 &&
+        6 * c.scope <= c.L.rcap && s0 + s1 < (1 << 18)) {
+      if (!((c.rpres[CM] >> (a0.ring * 32 + slot0)) & 1ull)) return;   // no M wavefront at s0
+      overlap_lanes(c, a0, slot0, a1, s0, s1, slot1, forward, tl, pl, bp);
+      return;
+    }
+  }
   int pm0, pi0, pd0;
   const Wf m0 = get_wf(c, a0, s0, slot0, CM, &pm0);
   if (!pm0) return;
   const Wf i0 = get_wf(c, a0, s0, slot0, CI, &pi0);
   const Wf d0 = get_wf(c, a0, s0, slot0, CD, &pd0);
-  const int tl = a0.tlen, pl = a0.plen;
   // ring slots of a1 holding any component (null scores hold none)
   const uint64_t live1 = ((c.rpres[0] | c.rpres[1] | c.rpres[2]) >> (32 * a1.ring)) & 0xffffffffull;
-  if constexpr (K::kLds) {
-    if (3 * c.scope <= kWave && c.L.wcap <= kWave && 2 * 3 * c.scope * c.L.wcap <= 4096 &&
-        6 * c.scope <= c.L.rcap && s0 + s1 < (1 << 18)) {
-      overlap_lanes(c, a1, s0, s1, slot1, forward, m0, i0, d0, pi0, pd0, tl, pl, bp);
-      return;
-    }
-  }
   if constexpr (K::kNW == 1) {
     for (int i = 0; i < c.scope; ++i) {
       const int si = s1 - i;
       if (si < 0) break;
-      if (s0 + si - c.o >= bp.score) continue;
+      if (s0 + si - c.o >= bp.score || (si & c.gmask)) continue;
       const int sl = slot_back(slot1, i, c.scope);
       if (!((live1 >> sl) & 1ull)) continue;
       int p1;
