@@ -469,7 +469,7 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
     kq = c.L.data + (hl * 3 + cq) * c.L.wcap;
     pm = sel3u(c.rpres, cq);
-  } else {
+  } else if constexpr (K::kLds) {
     hl = sq;
     pbit = sq & (kWave - 1);
     const int hb = (hl & (kWave - 1)) << 2;
@@ -480,6 +480,8 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
     kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
     pm = sel3u(c.bpres, cq);
+  } else {                      // global workspace: base-case headers are in memory (base_inputs)
+    hl = 0; pbit = 0; lq = 1; hq = -1; kq = 0; pm = 0;
   }
   const bool present = q < 4 && sq >= 0 && ((pm >> pbit) & 1ull);
   const bool nonempty = present && lq <= hq;
@@ -530,7 +532,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   Wf mm, mo, ie, de;
   int lo = 0, hi = 0;
   bool has_i = false, has_d = false;
-  if constexpr (K::kLds) {
+  if (K::kLds || d.ring >= 0) {
     if (!null) null = !gather_inputs(c, d, s, slot, mm, mo, ie, de, lo, hi, has_i, has_d);
   } else {
     if (!null) {
