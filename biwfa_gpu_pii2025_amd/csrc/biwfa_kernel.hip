@@ -344,6 +344,13 @@ DI void set_hdr(Ctx<K>& c, const Dir& d, int score, int slot, int comp, int lo, 
     c.pv[h] = lo;
     c.pv[h + 1] = hi;
     c.pv[h + 2] = kb;
+    // register copy of the last 64 scores (lane score % 64) for the steps'
+    // input gather; the backtrace reads the memory copy.  It reuses the ring
+    // header registers, which no base case needs (a breakpoint search starts
+    // by clearing its rings' presence bits).
+    lane_set(c.rlo, comp, c.lane, score & (kWave - 1), lo);
+    lane_set(c.rhi, comp, c.lane, score & (kWave - 1), hi);
+    lane_set(c.bkb, comp, c.lane, score & (kWave - 1), kb);
   }
 }
 
@@ -413,29 +420,6 @@ DI void null_step(Ctx<K>& c, Dir& d, int s, int slot) {
   set_hdr(c, d, s, slot, CD, kAbsent, 0, 0);
 }
 
-// The four input wavefronts of a base-case step held in memory (global
-// workspace): lane j loads the header of input j -- M[s-x], M[s-o-e], I1[s-e],
-// D1[s-e] -- so the four header loads overlap.  Same result as get_in.
-template <class K>
-DI void base_inputs(const Ctx<K>& c, int s, Wf& mm, Wf& mo, Wf& ie, Wf& de) {
-  const int l = c.lane;
-  const int sc = s - ((l == 0) ? c.x : (l == 1) ? c.o + c.e : c.e);
-  const int comp = (l <= 1) ? CM : (l == 2) ? CI : CD;
-  int lo = kAbsent, hi = 0, kb = 0;
-  if (l < 4 && sc >= 0) {
-    const int h = c.L.bhdr + (sc * 3 + comp) * 3;
-    lo = c.pv[h];
-    hi = c.pv[h + 1];
-    kb = c.pv[h + 2];
-  }
-  Wf w[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int wl = rdl(lo, j), wh = rdl(hi, j);
-    w[j] = (wl == kAbsent || wl > wh) ? wf_null() : Wf{wl, wh, rdl(kb, j), -1};
-  }
-  mm = w[0]; mo = w[1]; ie = w[2]; de = w[3];
-}
 
 // The four inputs of a step -- M[s-x], M[s-o-e], I1[s-e], D1[s-e] -- and the
 // limits they give (wavefront_compute_limits_input, wavefront_compute.c:40-86),
@@ -480,8 +464,21 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
     kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
     pm = sel3u(c.bpres, cq);
-  } else {                      // global workspace: base-case headers are in memory (base_inputs)
-    hl = 0; pbit = 0; lq = 1; hq = -1; kq = 0; pm = 0;
+  } else {
+    // global workspace: base-case headers of the last 64 scores in register
+    // lanes (lane score % 64, lo = kAbsent when absent; see set_hdr); a
+    // step's inputs are at most scope-1 < 64 scores back and every score in
+    // between was written
+    hl = sq;
+    const int hb = (hl & (kWave - 1)) << 2;
+    lq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rlo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rlo[CI]),
+               __builtin_amdgcn_ds_bpermute(hb, c.rlo[CD]), cq);
+    hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
+               __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
+    kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
+               __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
+    pbit = 0;
+    pm = lq != kAbsent ? 1ull : 0ull;
   }
   const bool present = q < 4 && sq >= 0 && ((pm >> pbit) & 1ull);
   const bool nonempty = present && lq <= hq;
@@ -532,31 +529,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   Wf mm, mo, ie, de;
   int lo = 0, hi = 0;
   bool has_i = false, has_d = false;
-  if (K::kLds || d.ring >= 0) {
-    if (!null) null = !gather_inputs(c, d, s, slot, mm, mo, ie, de, lo, hi, has_i, has_d);
-  } else {
-    if (!null) {
-      if (d.ring < 0) {
-        base_inputs(c, s, mm, mo, ie, de);
-      } else {
-        mm = get_in(c, d, s - c.x, slot_back(slot, c.x, sc), CM);
-        mo = get_in(c, d, s - c.o - c.e, slot_back(slot, c.o + c.e, sc), CM);
-        ie = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CI);
-        de = get_in(c, d, s - c.e, slot_back(slot, c.e, sc), CD);
-      }
-      null = mm.lo > mm.hi && mo.lo > mo.hi && ie.lo > ie.hi && de.lo > de.hi;
-    }
-    if (!null) {
-      const bool nmo = mo.lo > mo.hi, nie = ie.lo > ie.hi, nde = de.lo > de.hi;
-      // wavefront_compute_limits_input (wavefront_compute.c:40-86)
-      lo = mm.lo; hi = mm.hi;
-      lo = imin(lo, mo.lo - 1); hi = imax(hi, mo.hi + 1);
-      lo = imin(lo, ie.lo + 1); hi = imax(hi, ie.hi + 1);
-      lo = imin(lo, de.lo - 1); hi = imax(hi, de.hi - 1);
-      has_i = !nmo || !nie;
-      has_d = !nmo || !nde;
-    }
-  }
+  if (!null) null = !gather_inputs(c, d, s, slot, mm, mo, ie, de, lo, hi, has_i, has_d);
   if (null) {                                   // null step (wavefront_compute_affine.c:245-251)
     if (K::kLds && (s & c.gmask) != 0) {
       // A score that is not a multiple of g is never read in a single-wave
