@@ -98,6 +98,26 @@ DI int wave_max_nonneg(int v) {
   return rdl(v, kWave - 1);
 }
 
+// Max / min over the wave of any values (DPP row scans, then row broadcasts).
+DI int wave_max_any(int v) {
+  v = imax(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = imax(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = imax(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = imax(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = imax(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = imax(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return rdl(v, kWave - 1);
+}
+DI int wave_min_any(int v) {
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = imin(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return rdl(v, kWave - 1);
+}
+
 // A wavefront component as seen by readers: offsets for k in [lo,hi] live at
 // ws[base + (k & mask)] (mask = wcap-1 in a ring, -1 in the arena).  lo > hi is
 // the null wavefront (WFA2's wavefront_null has lo=1, hi=-1).
@@ -639,13 +659,25 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
         if (k == k_end) endv = (ce == CM) ? mx : (ce == CI) ? ins1[g] : del1[g];
       }
       // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
-      const uint64_t b_m = __ballot(live);
-      const uint64_t b_i = __ballot(act && (uint32_t)ins1[g] <= tl && (uint32_t)(ins1[g] - k) <= pl);
-      const uint64_t b_d = __ballot(act && (uint32_t)del1[g] <= tl && (uint32_t)(del1[g] - k) <= pl);
-      if (b_m) { if (fm == INT_MAX) fm = kb + __builtin_ctzll(b_m); lm = kb + 63 - __builtin_clzll(b_m); }
-      if (b_i) { if (fi == INT_MAX) fi = kb + __builtin_ctzll(b_i); li = kb + 63 - __builtin_clzll(b_i); }
-      if (b_d) { if (fd == INT_MAX) fd = kb + __builtin_ctzll(b_d); ld = kb + 63 - __builtin_clzll(b_d); }
+      const bool in_i = act && (uint32_t)ins1[g] <= tl && (uint32_t)(ins1[g] - k) <= pl;
+      const bool in_d = act && (uint32_t)del1[g] <= tl && (uint32_t)(del1[g] - k) <= pl;
+      if constexpr (K::kLds) {
+        const uint64_t b_m = __ballot(live), b_i = __ballot(in_i), b_d = __ballot(in_d);
+        if (b_m) { if (fm == INT_MAX) fm = kb + __builtin_ctzll(b_m); lm = kb + 63 - __builtin_clzll(b_m); }
+        if (b_i) { if (fi == INT_MAX) fi = kb + __builtin_ctzll(b_i); li = kb + 63 - __builtin_clzll(b_i); }
+        if (b_d) { if (fd == INT_MAX) fd = kb + __builtin_ctzll(b_d); ld = kb + 63 - __builtin_clzll(b_d); }
+      } else {
+        // per-lane extremes, reduced once after the loop: no scalar work per chunk
+        fm = imin(fm, live ? k : INT_MAX); lm = imax(lm, live ? k : INT_MIN);
+        fi = imin(fi, in_i ? k : INT_MAX); li = imax(li, in_i ? k : INT_MIN);
+        fd = imin(fd, in_d ? k : INT_MAX); ld = imax(ld, in_d ? k : INT_MIN);
+      }
     }
+  }
+  if constexpr (!K::kLds) {
+    fm = wave_min_any(fm); lm = wave_max_any(lm);
+    fi = wave_min_any(fi); li = wave_max_any(li);
+    fd = wave_min_any(fd); ld = wave_max_any(ld);
   }
   // offset at k_end: held by lane (k_end - lo) % 64 of the wave owning its chunk
   if (k_end >= lo && k_end <= hi) endu = rdl(endv, (k_end - lo) & (kWave - 1));
