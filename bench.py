@@ -10,9 +10,10 @@ x 150 bp, 2% edits, x=4 o=6 e=2, length-sorted schedule).
 
 Prints one JSON line.  Besides the contract fields it carries
   roofline      -- HBM roofline of the dominant kernel (algorithmic bytes / kernel time),
-  issue_roofline -- scalar-instruction issue rate of that kernel (the bound it runs
-                   against: one SALU instruction per CU per clock), from the kernel's
-                   per-pair SQ counts in profiles/ and the live kernel time,
+  issue_roofline -- instruction issue rate of that kernel (the bound it runs against:
+                   one SALU and one VALU wave instruction per CU per clock; the
+                   busier of the two is reported), from the kernel's per-pair SQ
+                   counts in profiles/ and the live kernel time,
   int_roofline  -- the algorithmic integer work (cells and compares) against the
                    int32 VALU peak,
   cpu_baseline  -- WFA2-lib BiWFA (built from the reference sources) with OpenMP
@@ -310,10 +311,11 @@ def main():
     if sq is not None:
         salu_g = sq["salu_per_pair"] * n / main_s / 1e9
         valu_g = sq["valu_per_pair"] * n / main_s / 1e9
+        top = max(salu_g, valu_g)
         out["issue_roofline"] = {
-            "bound": "salu-issue", "achieved": round(salu_g, 1), "peak": ISSUE_PEAK_G,
-            "unit": "G instr/s", "frac": round(salu_g / ISSUE_PEAK_G, 4),
-            "valu_frac": round(valu_g / ISSUE_PEAK_G, 4),
+            "bound": "valu-issue" if valu_g >= salu_g else "salu-issue", "achieved": round(top, 1),
+            "peak": ISSUE_PEAK_G, "unit": "G instr/s", "frac": round(top / ISSUE_PEAK_G, 4),
+            "salu_frac": round(salu_g / ISSUE_PEAK_G, 4), "valu_frac": round(valu_g / ISSUE_PEAK_G, 4),
             "salu_per_pair": round(sq["salu_per_pair"], 1), "valu_per_pair": round(sq["valu_per_pair"], 1),
             "counts_from": "profiles/biwfa_persistent_lds_C2_1M_sq_counters.md",
         }
