@@ -538,7 +538,7 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
 // computation as compute-all-then-extend.  Returns 1 when the aligner is done
 // (end reached, or END_UNREACHABLE after scope+1 null steps); *max_ak gets the
 // maximum antidiagonal 2*offset-k over the extended cells.
-template <class K>
+template <class K, bool kMax = true>
 DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   *max_ak = 0;
   const int sc = c.scope;
@@ -619,7 +619,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     if (b_d) { fd = lo + __builtin_ctzll(b_d); ld = lo + 63 - __builtin_clzll(b_d); }
     const int ev = (ce == CM) ? mx : (ce == CI) ? ins1 : del1;
     if (k_end >= lo && k_end <= hi) endu = rdl(ev, k_end - lo);
-    lmax = wave_max_nonneg(ak);
+    if constexpr (kMax) lmax = wave_max_nonneg(ak);
   } else {
   // A wave takes kG of its chunks at a time and issues all their loads before
   // any store: the stores of one chunk could alias the next chunk's inputs as
@@ -681,7 +681,8 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   }
   // offset at k_end: held by lane (k_end - lo) % 64 of the wave owning its chunk
   if (k_end >= lo && k_end <= hi) endu = rdl(endv, (k_end - lo) & (kWave - 1));
-  lmax = wave_max_nonneg(lmax);
+  if constexpr (kMax) lmax = wave_max_nonneg(lmax);
+  else lmax = 0;
   }
   if constexpr (K::kNW > 1) {   // combine the team's waves
     int nfm = -fm, nfi = -fi, nfd = -fd;
@@ -1093,7 +1094,7 @@ DI int find_breakpoint(Ctx<K>& c, Dir& f, Dir& r, BP& bp) {
       if (sf + min_r - c.o >= bp.score) break;
       overlap(c, f, r, sf, qf, sr, qr, true, bp);
       ++sr; qr = next_slot(qr, c.scope);
-      const int rq = compute_extend(c, r, sr, qr, &ak);
+      const int rq = compute_extend<K, false>(c, r, sr, qr, &ak);
       if (c.spill) return kStCapacity;
       if (rq) return r.status;
     }
@@ -1102,7 +1103,7 @@ DI int find_breakpoint(Ctx<K>& c, Dir& f, Dir& r, BP& bp) {
     if (sf + sr > c.guard) return kStGuard;
     overlap(c, r, f, sr, qr, sf, qf, false, bp);
     ++sf; qf = next_slot(qf, c.scope);
-    const int fq = compute_extend(c, f, sf, qf, &ak);
+    const int fq = compute_extend<K, false>(c, f, sf, qf, &ak);
     if (c.spill) return kStCapacity;
     if (fq) return f.status;
     if (sr + sf >= c.max_steps) return kStMaxSteps;
@@ -1316,7 +1317,7 @@ DI int base_case(Ctx<K>& c, int pb, int pe, int tb, int te, int cb, int ce, int*
   int done = extend_step(c, b, 0, 0, &ak);
   while (!done) {
     ++score;
-    done = compute_extend(c, b, score, 0, &ak);
+    done = compute_extend<K, false>(c, b, score, 0, &ak);
     if (c.spill) return kStCapacity;
     if (score >= c.max_steps) return kStUnattainable;   // reported as unattainable by the base
     if (score > guard) return kStGuard;
