@@ -157,6 +157,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=0, metavar="N",
                     help="tune launch shapes on an N-pair sample of the workload before timing")
     ap.add_argument("--tuning", default=None, help="JSON launch tuning (bw_tuning_t fields)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the captured launch sequence (bw_set_graph); per-tier times read 0")
     ap.add_argument("--compare-schedules", action="store_true",
                     help="train the cost model, then time the plain length-sorted and the learned "
                          "schedule on the same batch (value = learned)")
@@ -192,6 +194,8 @@ def main():
     aligner = bw.WavefrontAligner(penalties=bw.configs.PENALTIES, schedule=sched)
     if args.tuning:
         aligner.tuning = json.loads(args.tuning)
+    if args.graph:
+        aligner.graph = True
     sample = None
     if args.autotune > 0:
         t0 = time.time()
@@ -287,6 +291,7 @@ def main():
                 "learned (cost model)" if args.compare_schedules else
                 "length-sorted" if args.schedule == "length" else "input-order"),
             "pairs": n,
+            "graph_replay": bool(args.graph),
             "pattern_length": [int(packed.p_len.min()), int(packed.p_len.max())],
             "penalties": list(bw.configs.PENALTIES),
         },

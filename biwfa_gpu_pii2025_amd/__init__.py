@@ -128,6 +128,7 @@ def lib():
         L.bw_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch), ctypes.c_void_p, ctypes.c_int,
                                   ctypes.POINTER(_Tuning)]
         L.bw_set_progress.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.bw_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.bw_dataset_lengths.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_generate.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]
@@ -429,6 +430,18 @@ class WavefrontAligner:
 
     def upload(self, packed):
         return DeviceBatch(self, packed)
+
+    # -- graph mode: replay a captured launch sequence (speed only; results never change) --
+    @property
+    def graph(self):
+        return bool(getattr(self, "_graph", False))
+
+    @graph.setter
+    def graph(self, enable):
+        st = self._L.bw_set_graph(self._h, 1 if enable else 0)
+        if st != 0:
+            raise BiwfaError("bw_set_graph failed (%d)" % st)
+        self._graph = bool(enable)
 
     # -- launch tuning (speed only; results never change) --
     @property

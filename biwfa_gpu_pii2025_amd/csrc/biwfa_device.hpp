@@ -103,7 +103,8 @@ struct KParams {
   int32_t task;                  // 1: shared-task recursion
   int32_t* items;                // item_cap x kItemInts
   int32_t item_cap;
-  int32_t epoch;                 // ready-flag value of this launch (never 0)
+  int32_t epoch;                 // tier index; the ready-flag value is (*calls << 3) | epoch
+  const int32_t* calls;          // call counter, advanced by biwfa_reset (never 0 after it)
   int32_t* qctl;                 // this tier's {queue, count, head, tail, in-flight task pairs, error}
   int32_t* pend;                 // per pair: unfinished sub-problems
   unsigned long long* pfail;     // per pair: leftmost failure, (pb+tb) << 16 | -status

@@ -177,6 +177,7 @@ struct Ctx {
   int spill;      // a workspace capacity was exceeded
   int cap_err;    // the pair's output run capacity was exceeded
   int guard;      // score bound of the current sub-problem (termination guard)
+  int epoch;      // ready-flag value of this launch: call count << 3 | tier (never 0)
 };
 
 // Upper bound of the optimal score of a plen x tlen sub-problem plus the
@@ -1532,7 +1533,7 @@ DI int publish(Ctx<K>& c, const KParams& p, const Sub& s) {
       it[1] = s.pair; it[2] = s.pb; it[3] = s.pe; it[4] = s.tb; it[5] = s.te;
       it[6] = s.cb; it[7] = s.ce; it[8] = s.rem; it[9] = s.lvl;
       __threadfence();
-      __hip_atomic_store(it, p.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(it, c.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       ok = 1;
     }
   }
@@ -1779,7 +1780,7 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
       int ready = 1, unused = 0;
       if (c.leader) {
         const uint64_t t0 = wall_clock64();
-        while (atomic_ld(it) != p.epoch) {
+        while (atomic_ld(it) != c.epoch) {
           if (wall_clock64() - t0 > kReadyTicks) { ready = 0; break; }
           __builtin_amdgcn_s_sleep(2);
         }
@@ -1826,6 +1827,7 @@ void biwfa_persistent(KParams p) {
   c.max_steps = p.max_steps;
   c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
   c.gmask = p.gmask;
+  c.epoch = (uni(*(volatile const int32_t*)p.calls) << 3) | p.epoch;
   const int count = uni(*(volatile const int32_t*)p.count);
   if constexpr (!K::kLds) {
     // The list is final at launch (the tier below has finished); with no pairs
@@ -1901,9 +1903,13 @@ void biwfa_persistent(KParams p) {
 
 // Sets every tier's {queue, count} counters to their initial values (a kernel
 // rather than a memcpy/memset so that the launch sequence can be captured in a graph).
+// Sets every tier's counters for a new launch sequence and advances the call
+// counter at counters[n] (the ready-flag epochs of the sequence derive from
+// it, on the device, so a captured sequence replays with fresh epochs).
 __global__ void biwfa_reset(int32_t* counters, const int32_t* init, int n) {
   const int i = threadIdx.x;
   if (i < n) counters[i] = init[i];
+  if (i == 0) counters[n] = (counters[n] + 1) & 0x0fffffff;
 }
 
 // Copies each pair's runs into the compacted layout (one wave per pair).

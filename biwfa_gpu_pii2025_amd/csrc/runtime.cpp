@@ -186,6 +186,9 @@ struct bw_dbatch {
   int32_t* d_pflag = nullptr;
   int64_t gws_bytes = 0;
   std::vector<int32_t> tier_count;   // initial pairs per tier (last schedule)
+  // graph mode: the launch sequence captured for one aligner, replayed on later calls
+  hipGraphExec_t gexec = nullptr;
+  const void* g_aligner = nullptr;
 };
 
 constexpr int kMaxTiers = 8;
@@ -204,8 +207,10 @@ struct bw_aligner {
   bw_tuning_t tuning{};
   bw_progress_fn progress = nullptr;
   void* progress_user = nullptr;
-  int32_t* d_counters = nullptr;   // per tier: {queue, count, head, tail, in-flight task pairs, error}
-  int32_t epoch_calls = 0;         // launches so far (ready-flag epochs)
+  int32_t* d_counters = nullptr;   // per tier: {queue, count, head, tail, in-flight task pairs, error},
+                                   // then the call counter the ready-flag epochs derive from
+  int graph = 0;                   // replay a captured launch sequence (bw_set_graph)
+  bool last_graph = false;         // the last sequence was a graph replay (no per-tier events)
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
   // last single-pair result
   int status = 0, score = 0, wf_score = 0;
@@ -282,7 +287,8 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
   a->lds_per_block_max = prop.sharedMemPerBlock;
   a->schedule = attr.schedule;
   if (hipSetDevice(a->dev) != hipSuccess || hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||
-      dalloc(&a->d_counters, bw::kQctlInts * kMaxTiers) != hipSuccess) {
+      dalloc(&a->d_counters, bw::kQctlInts * kMaxTiers + 1) != hipSuccess ||
+      hipMemset(a->d_counters, 0, sizeof(int32_t) * (bw::kQctlInts * kMaxTiers + 1)) != hipSuccess) {
     set_error("stream / counter allocation failed");
     delete a;
     return nullptr;
@@ -307,6 +313,7 @@ void bw_aligner_delete(bw_aligner_t* a) {
 
 void bw_dbatch_free(bw_dbatch_t* b) {
   if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   dfree(b->d_words); dfree(b->d_pwoff); dfree(b->d_plen); dfree(b->d_twoff); dfree(b->d_tlen);
   dfree(b->d_lists); dfree(b->d_init); dfree(b->d_score); dfree(b->d_wf); dfree(b->d_status);
   dfree(b->d_nruns); dfree(b->d_runs); dfree(b->d_runs_off); dfree(b->d_gws);
@@ -601,19 +608,19 @@ static bw::KParams base_params(const bw_aligner_t* a, const bw_dbatch_t* b) {
   return p;
 }
 
-int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
-  if (!a || !b) return BW_STATUS_INVALID_ARG;
-  HIP_TRY(hipSetDevice(a->dev));
-  if (b->n == 0) return 0;
+// The per-call launch sequence: reset of the tiers' counters (which also
+// advances the call counter), then one persistent launch per tier.  Tier
+// boundary events are recorded unless the sequence is being captured.
+static int launch_sequence(bw_aligner_t* a, bw_dbatch_t* b, bool events) {
   hipStream_t s = a->stream;
   const int nt = (int)b->tiers.size();
-  HIP_TRY(hipEventRecord(a->ev[0], s));
   hipLaunchKernelGGL(bw::biwfa_reset, dim3(1), dim3(64), 0, s, a->d_counters, b->d_init,
                      bw::kQctlInts * kMaxTiers);
-  ++a->epoch_calls;
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(a->ev[1], s));
-  HIP_TRY(hipEventRecord(a->tev[0], s));
+  if (events) {
+    HIP_TRY(hipEventRecord(a->ev[1], s));
+    HIP_TRY(hipEventRecord(a->tev[0], s));
+  }
   for (int t = 0; t < nt; ++t) {
     const Tier& T = b->tiers[t];
     bw::KParams p = base_params(a, b);
@@ -624,7 +631,8 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
     p.task = 1;
     p.items = b->d_items;
     p.item_cap = b->item_cap;
-    p.epoch = (int32_t)((((uint32_t)a->epoch_calls & 0x0fffffffu) << 3) | (uint32_t)t);
+    p.epoch = t;
+    p.calls = a->d_counters + bw::kQctlInts * kMaxTiers;
     p.pend = b->d_pend;
     p.pfail = b->d_pfail;
     p.pflag = b->d_pflag;
@@ -643,11 +651,61 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
       HIP_TRY(hipLaunchKernel(team_kernel(T.nw), dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), args,
                               T.dyn_lds, s));
     }
-    HIP_TRY(hipEventRecord(a->tev[t + 1], s));
-    if (t == 0) HIP_TRY(hipEventRecord(a->ev[2], s));
+    if (events) {
+      HIP_TRY(hipEventRecord(a->tev[t + 1], s));
+      if (t == 0) HIP_TRY(hipEventRecord(a->ev[2], s));
+    }
   }
+  return 0;
+}
+
+int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
+  if (!a || !b) return BW_STATUS_INVALID_ARG;
+  HIP_TRY(hipSetDevice(a->dev));
+  if (b->n == 0) return 0;
+  hipStream_t s = a->stream;
+  const int nt = (int)b->tiers.size();
+  if (a->graph) {
+    if (!b->gexec || b->g_aligner != a) {
+      if (b->gexec) {
+        HIP_TRY(hipGraphExecDestroy(b->gexec));
+        b->gexec = nullptr;
+      }
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int st = launch_sequence(a, b, false);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &g);
+      if (st != 0) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      HIP_TRY(e);
+      const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_TRY(ei);
+      b->g_aligner = a;
+    }
+    HIP_TRY(hipEventRecord(a->ev[0], s));
+    HIP_TRY(hipEventRecord(a->ev[1], s));
+    HIP_TRY(hipGraphLaunch(b->gexec, s));
+    HIP_TRY(hipEventRecord(a->ev[2], s));
+    HIP_TRY(hipEventRecord(a->ev[3], s));
+    a->last_ntiers = nt;
+    a->last_graph = true;
+    return 0;
+  }
+  HIP_TRY(hipEventRecord(a->ev[0], s));
+  const int st = launch_sequence(a, b, true);
+  if (st != 0) return st;
   HIP_TRY(hipEventRecord(a->ev[3], s));
   a->last_ntiers = nt;
+  a->last_graph = false;
+  return 0;
+}
+
+int bw_set_graph(bw_aligner_t* a, int enable) {
+  if (!a) return BW_STATUS_INVALID_ARG;
+  a->graph = enable ? 1 : 0;
   return 0;
 }
 
@@ -661,7 +719,7 @@ int bw_synchronize(bw_aligner_t* a) {
   if (hipEventElapsedTime(&spill_ms, a->ev[2], a->ev[3]) == hipSuccess) a->last_spill_ms = spill_ms;
   for (int t = 0; t < a->last_ntiers; ++t) {
     float ms = 0.f;
-    a->tier_ms[t] = hipEventElapsedTime(&ms, a->tev[t], a->tev[t + 1]) == hipSuccess ? ms : 0.f;
+    a->tier_ms[t] = !a->last_graph && hipEventElapsedTime(&ms, a->tev[t], a->tev[t + 1]) == hipSuccess ? ms : 0.f;
   }
   int32_t cnt[bw::kQctlInts * kMaxTiers];
   HIP_TRY(hipMemcpy(cnt, a->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));

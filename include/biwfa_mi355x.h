@@ -247,6 +247,16 @@ int bw_autotune(bw_aligner_t* aligner, const bw_batch_t* sample, const float* pr
 typedef void (*bw_progress_fn)(const char* message, void* user);
 int bw_set_progress(bw_aligner_t* aligner, bw_progress_fn fn, void* user);
 
+/* Graph mode (enable != 0).  The first bw_dbatch_align of a device batch on this
+ * aligner then captures the call's launch sequence -- the counter reset and one
+ * persistent launch per tier; the recursion itself runs on the device -- into a
+ * HIP graph, and later calls on the same batch replay it as one launch.  Results
+ * are identical.  bw_last_tier_timing reports zeros after a replay (no events
+ * between the tiers); bw_last_timing reports the whole sequence as the main part.
+ * Replaces the reference's host loop of one launch per 100-pair chunk
+ * (src/biWFA.cu:935-1004). */
+int bw_set_graph(bw_aligner_t* aligner, int enable);
+
 /* ---------------- (v) learned cost model ---------------- */
 
 /* No counterpart in the reference: a 4 -> 64 -> 64 -> 1 ReLU MLP (BF16 weights,

@@ -320,3 +320,43 @@ def test_global_tiers_step_cap_and_score_only(aligners, oracle):
     assert set(int(s) for s in res.status) == {0}
     assert np.array_equal(res.wf_score, ref.wf_score)
     assert np.array_equal(res.score, ref.alg_score)
+
+
+def test_graph_replay_matches_eager():
+    """Graph mode: the captured launch sequence replayed several times, with other
+    eager work and synchronisations in between, gives the eager results every time
+    (scores, CIGARs, statuses), on a batch that spans the LDS tier and the global
+    tiers' shared recursion tasks."""
+    _say("graph replay")
+    ds = bw.generate_dataset(bw.CONFIGS["C4"], num_pairs=160)
+    short = bw.generate_dataset(bw.CONFIGS["C2"], num_pairs=200)
+    packed = bw.pack_pairs(ds.pairs() + short.pairs())
+    other = bw.generate_dataset(bw.CONFIGS["C1"], num_pairs=300).pack()
+    eager = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
+    ref = eager.align_batch(packed)
+    assert set(int(s) for s in ref.status) == {0}
+    ref_other = eager.align_batch(other)
+    g = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
+    g.graph = True
+    assert g.graph
+    db = g.upload(packed)
+    for rep in range(4):
+        db.align()
+        res = db.download()
+        assert np.array_equal(res.status, ref.status), rep
+        assert np.array_equal(res.score, ref.score), rep
+        assert np.array_equal(res.cigar_offset, ref.cigar_offset), rep
+        assert np.array_equal(res.cigar_runs, ref.cigar_runs), rep
+        assert sum(g.tier_pairs()) >= len(packed)
+        # eager work of the program between replays, on the same device
+        assert np.array_equal(eager.align_batch(other).score, ref_other.score)
+        _say("replay %d identical" % rep)
+    # switching graph mode off returns to eager launches with per-tier timings
+    g.graph = False
+    db.align()
+    res = db.download()
+    assert np.array_equal(res.score, ref.score) and np.array_equal(res.cigar_runs, ref.cigar_runs)
+    assert sum(g.tier_timing()) > 0
+    db.free()
+    g.close()
+    eager.close()
