@@ -47,8 +47,6 @@ enum { CM = 0, CI = 1, CD = 2 };
 constexpr int kAbsent = INT_MAX;   // header lo of a component WFA2 has not allocated
 // uint32 run op codes (SAM encoding with mismatches shown, cigar.c:38-56)
 constexpr int kOpM = 7, kOpX = 8, kOpI = 1, kOpD = 2;
-// Base-case headers held in registers (one lane per score) in the LDS kernel.
-constexpr int kRegScores = kWave;
 
 DI int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 DI int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
@@ -319,13 +317,6 @@ DI Wf get_wf(const Ctx<K>& c, const Dir& d, int score, int slot, int comp, int* 
   return Wf{lo, hi, kb, -1};
 }
 
-template <class K>
-DI Wf get_in(const Ctx<K>& c, const Dir& d, int score, int slot, int comp) {
-  int present;
-  Wf w = get_wf(c, d, score, slot, comp, &present);
-  if (w.lo > w.hi) return wf_null();
-  return w;
-}
 
 DI void lane_set(int (&a)[3], int comp, int lane, int j, int v) {
   const bool me = lane == j;
@@ -444,14 +435,18 @@ DI void null_step(Ctx<K>& c, Dir& d, int s, int slot) {
 
 // The four inputs of a step -- M[s-x], M[s-o-e], I1[s-e], D1[s-e] -- and the
 // limits they give (wavefront_compute_limits_input, wavefront_compute.c:40-86),
-// for a single-wave LDS aligner, lane-parallel: lane q < 4 fetches input q's
-// header from the header lanes (ds_bpermute), tests its presence, normalises an
-// absent or empty input to the null wavefront (lo=1, hi=-1, which still takes
-// part in the limits as in WFA2), and the limits are reduced over lanes 0..3.
-// Same results as four get_in calls, with the per-input control done in vector
-// lanes instead of one scalar chain per input.  Returns false when all four
-// inputs are null.  Ring inputs use mask wcap-1, base-case inputs mask -1; a
-// null input's base is chosen so that a clamped read stays in the workspace.
+// lane-parallel: lane q < 4 fetches input q's header from the header lanes
+// (ds_bpermute), tests its presence, normalises an absent or empty input to the
+// null wavefront (lo=1, hi=-1, which still takes part in the limits as in WFA2),
+// and the limits are reduced over lanes 0..3.  Same results as reading the four
+// inputs one by one (wavefront_compute_get_*wavefront, wavefront_compute.c:266-305)
+// with empty inputs taken as null, with the per-input control in vector lanes
+// instead of one scalar chain per input.  Ring headers live in the ring header
+// lanes; base-case headers in the LDS kernel's per-score lanes, and in the
+// global kernels in a register copy of the last 64 scores (see set_hdr).
+// Returns false when all four inputs are null.  Ring inputs use mask wcap-1,
+// base-case inputs mask -1; a null input's base keeps a clamped read inside the
+// workspace.
 template <class K>
 DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf& mo, Wf& ie, Wf& de,
                       int& lo, int& hi, bool& has_i, bool& has_d) {
@@ -542,7 +537,6 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
 template <class K, bool kMax = true>
 DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   *max_ak = 0;
-  const int sc = c.scope;
   if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return 0; }
   bool null = (s & c.gmask) != 0;
   // With penalties that share a power-of-two factor g, only multiples of g are
