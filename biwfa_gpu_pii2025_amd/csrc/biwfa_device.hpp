@@ -24,7 +24,10 @@ constexpr int kStGuard = -405;       // termination guard tripped (never in a co
 constexpr int kMaxScope = 32;          // max(o+e, x)+1 supported by the ring layout
 constexpr int kWave = 64;              // gfx950 wavefront size
 constexpr int kWavesPerBlock = 4;      // independent single-wave aligners per workgroup
-constexpr int kMaxTeam = 16;           // waves cooperating on one pair (one workgroup)
+constexpr int kMaxTeam = 16;
+// LDS-tier workspace capacities (compile-time in its kernel): base-case scores
+// (one header lane each), backtrace runs, recursion stack entries
+constexpr int kLdsHcap = 64, kLdsRcap = 128, kLdsScap = 16;           // waves cooperating on one pair (one workgroup)
 constexpr int kRedSlots = 3 * kMaxScope;   // team reduction values per wave
 constexpr int kSubInts = 10;           // recursion stack entry (a sub-problem)
 constexpr int kItemInts = 12;          // shared recursion task: ready flag + sub-problem
@@ -45,7 +48,7 @@ struct Cfg {
 // base-case headers, the backtrace buffer and the recursion stack, so that the
 // wave-uniform bookkeeping never needs cross-wave synchronisation.
 struct WsLayout {
-  int data;         // ring data (2 x scope x 3 x wcap)  UNION  base-case arena
+  int data;         // ring data (2 x scope x 3 x wcap)  UNION  base-case arena (offset 0)
   int wcap;         // ring width capacity (power of two)
   int arena_cap;    // ints available at `data` for the base-case arena
   int priv;         // first private region

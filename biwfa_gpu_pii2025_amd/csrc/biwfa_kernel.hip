@@ -271,9 +271,16 @@ DI int team_min_slot(const Ctx<K>& c, const int32_t* buf, int j) {
 // ---------------------------------------------------------------------------
 // headers
 
+// Start of the ring / base-case arena in the workspace: offset 0 by
+// construction, a compile-time constant in the LDS kernel (every uniform value
+// the LDS kernel keeps out of its scalar registers helps: it runs close to the
+// scalar register limit).
+template <class K>
+DI int ws_data(const Ctx<K>& c) { return K::kLds ? 0 : c.L.data; }
+
 template <class K>
 DI int ring_base(const Ctx<K>& c, int ring, int slot, int comp) {
-  return c.L.data + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
+  return ws_data(c) + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
 }
 
 // Presence bit of (ring slot | base score) for component `comp`: ring slots use
@@ -467,7 +474,7 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
                __builtin_amdgcn_ds_bpermute(hb, c.rlo[CD]), cq);
     hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
-    kq = c.L.data + (hl * 3 + cq) * c.L.wcap;
+    kq = ws_data(c) + (hl * 3 + cq) * c.L.wcap;
     pm = sel3u(c.rpres, cq);
   } else if constexpr (K::kLds) {
     hl = sq;
@@ -537,7 +544,7 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
 template <class K, bool kMax = true>
 DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   *max_ak = 0;
-  if (d.ring < 0 && s >= c.L.hcap) { c.spill = 1; return 0; }
+  if (d.ring < 0 && s >= (K::kLds ? kLdsHcap : c.L.hcap)) { c.spill = 1; return 0; }
   bool null = (s & c.gmask) != 0;
   // With penalties that share a power-of-two factor g, only multiples of g are
   // reachable scores: the other steps are null without looking at the inputs.
@@ -573,11 +580,11 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   } else {
     const int need = width * (1 + (int)has_i + (int)has_d);
     if (c.arena_top + need > c.L.arena_cap) { c.spill = 1; return 0; }
-    int top = c.L.data + c.arena_top;
+    int top = ws_data(c) + c.arena_top;
     bm = top - lo; top += width;
     bi = top - lo; if (has_i) top += width;
     bd = top - lo; if (has_d) top += width;
-    c.arena_top = top - c.L.data;
+    c.arena_top = top - ws_data(c);
     mask = -1;
   }
   const uint32_t tl = (uint32_t)d.tlen, pl = (uint32_t)d.plen;
@@ -888,8 +895,8 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
   const int j0 = a0.ring * c.scope + slot0;
   const int lo0 = pick3(rdl(c.rlo[CD], j0), rdl(c.rlo[CI], j0), rdl(c.rlo[CM], j0), cc);
   const int hi0 = pick3(rdl(c.rhi[CD], j0), rdl(c.rhi[CI], j0), rdl(c.rhi[CM], j0), cc);
-  const int b0 = c.L.data + (j0 * 3 + comp) * c.L.wcap;
-  const int b1 = c.L.data + (j * 3 + comp) * c.L.wcap;
+  const int b0 = ws_data(c) + (j0 * 3 + comp) * c.L.wcap;
+  const int b1 = ws_data(c) + (j * 3 + comp) * c.L.wcap;
   const bool p0 = (pres >> (a0.ring * 32 + slot0)) & 1ull;
   const int klo = imax(lo0, dk - hi1), khi = imin(hi0, dk - lo1);
   const bool ok = i < c.scope && si >= 0 && (si & c.gmask) == 0 && sc < bp.score && p0 &&
@@ -956,7 +963,7 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
   const int tl = a0.tlen, pl = a0.plen;
   if constexpr (K::kLds) {
     if (3 * c.scope <= kWave && c.L.wcap <= kWave && 2 * 3 * c.scope * c.L.wcap <= 4096 &&
-        6 * c.scope <= c.L.rcap && s0 + s1 < (1 << 18)) {
+        6 * c.scope <= kLdsRcap && s0 + s1 < (1 << 18)) {
       if (!((c.rpres[CM] >> (a0.ring * 32 + slot0)) & 1ull)) return;   // no M wavefront at s0
       overlap_lanes(c, a0, slot0, a1, s0, s1, slot1, forward, tl, pl, bp);
       return;
@@ -1144,7 +1151,7 @@ DI void rev_emit(Ctx<K>& c, RevRuns& rr, int op, int n) {
   if (n <= 0) return;
   if (op == rr.op) { rr.len += n; return; }
   if (rr.len > 0) {
-    if (rr.n >= c.L.rcap) { c.spill = 1; return; }
+    if (rr.n >= (K::kLds ? kLdsRcap : c.L.rcap)) { c.spill = 1; return; }
     if (c.lane == 0) c.pv[c.L.runs_tmp + rr.n] = (int)(((uint32_t)rr.len << 4) | (uint32_t)rr.op);
     rr.n++;
   }
@@ -1192,7 +1199,7 @@ DI long long bt_max_mem(const Ctx<K>& c, int score, int matrix, int k) {
   for (int j = 0; j < 5; ++j) {
     const int l = uni(lo[j]), u = uni(hi[j]), kk = k + dk[j];
     in[j] = en[j] && sj[j] >= 0 && l != kAbsent && l <= kk && kk <= u;
-    off[j] = c.ws[in[j] ? uni(kb[j]) + kk : c.L.data];
+    off[j] = c.ws[in[j] ? uni(kb[j]) + kk : ws_data(c)];
   }
   long long m = LLONG_MIN;
 #pragma unroll
@@ -1302,10 +1309,10 @@ DI int base_case(Ctx<K>& c, int pb, int pe, int tb, int te, int cb, int ce, int*
   if constexpr (K::kNW > 1) __syncthreads();
   // score 0: the begin component at k=0 (wavefront_aligner.c:311-383)
   c.arena_top = 1;
-  set_hdr(c, b, 0, 0, CM, cb == CM ? 0 : kAbsent, 0, c.L.data);
-  set_hdr(c, b, 0, 0, CI, cb == CI ? 0 : kAbsent, 0, c.L.data);
-  set_hdr(c, b, 0, 0, CD, cb == CD ? 0 : kAbsent, 0, c.L.data);
-  c.ws[c.L.data] = 0;
+  set_hdr(c, b, 0, 0, CM, cb == CM ? 0 : kAbsent, 0, ws_data(c));
+  set_hdr(c, b, 0, 0, CI, cb == CI ? 0 : kAbsent, 0, ws_data(c));
+  set_hdr(c, b, 0, 0, CD, cb == CD ? 0 : kAbsent, 0, ws_data(c));
+  c.ws[ws_data(c)] = 0;
   team_sync<K>();
   int score = 0, ak;
   const int guard = score_guard(c, b.plen, b.tlen);
@@ -1331,7 +1338,7 @@ struct Sub {
 };
 template <class K>
 DI void push(Ctx<K>& c, int& sp, const Sub& s) {
-  if (sp >= c.L.scap) { c.spill = 1; return; }
+  if (sp >= (K::kLds ? kLdsScap : c.L.scap)) { c.spill = 1; return; }
   int* e = c.pv + c.L.stack + sp * kSubInts;
   e[0] = s.pair; e[1] = s.pb; e[2] = s.pe; e[3] = s.tb; e[4] = s.te;
   e[5] = s.cb; e[6] = s.ce; e[7] = s.rem; e[8] = s.lvl;

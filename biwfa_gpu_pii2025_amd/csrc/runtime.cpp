@@ -363,7 +363,7 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
     t.seq_cap = tu.lds_seq_cap > 0 ? tu.lds_seq_cap : 512;
     const int seq_words = (t.seq_cap + 31) / 32 + 3;
     const int lds_wcap = tu.lds_ring_width > 0 ? tu.lds_ring_width : 32;
-    t.L = make_layout(scope, lds_wcap, 0, bw::kWave, 128, 16, seq_words, false, 1);
+    t.L = make_layout(scope, lds_wcap, 0, bw::kLdsHcap, bw::kLdsRcap, bw::kLdsScap, seq_words, false, 1);
     t.waves_per_block = bw::kWavesPerBlock;
     t.dyn_lds = (size_t)t.L.total * 4 * t.waves_per_block;
     int nb = 0;
