@@ -162,7 +162,8 @@ struct Ctx {
   int tph;        // its double-buffer phase
   WsLayout L;
   int x, o, e, scope, max_steps;
-  int um, ux, uo, ue;
+  int um, ux, uo, ue;   // user penalties (global kernels)
+  int upen;             // the same in lanes 0..3 (LDS kernel)
   int arena_top;
   // headers in registers: ring (lane = ring*scope + slot) and base case (lane = score)
   int rlo[3], rhi[3];
@@ -277,6 +278,14 @@ DI int team_min_slot(const Ctx<K>& c, const int32_t* buf, int j) {
 // scalar register limit).
 template <class K>
 DI int ws_data(const Ctx<K>& c) { return K::kLds ? 0 : c.L.data; }
+
+// Offsets in a wave's private region of the backtrace run buffer and the
+// recursion stack (compile-time in the LDS kernel: its region holds no
+// base-case headers, so the runs start at 0 and the stack after them).
+template <class K>
+DI int runs_off(const Ctx<K>& c) { return K::kLds ? 0 : c.L.runs_tmp; }
+template <class K>
+DI int stack_off(const Ctx<K>& c) { return K::kLds ? kLdsRcap : c.L.stack; }
 
 template <class K>
 DI int ring_base(const Ctx<K>& c, int ring, int slot, int comp) {
@@ -904,7 +913,7 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
   const uint64_t vm = __ballot(ok);
   if (!vm) return;
   const int nr = __builtin_popcountll(vm);
-  int32_t* const tab = c.pv + c.L.runs_tmp;
+  int32_t* const tab = c.pv + runs_off(c);
   if (ok) {
     const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(vm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)vm, 0));
     tab[2 * r] = (klo & 0xffff) | (khi << 16);
@@ -1117,6 +1126,14 @@ DI int find_breakpoint(Ctx<K>& c, Dir& f, Dir& r, BP& bp) {
 // ---------------------------------------------------------------------------
 // output runs
 
+// User penalties for the .alg score: scalar registers in the global kernels,
+// lanes 0..3 of one vector register in the LDS kernel (which runs close to its
+// scalar register limit; they are read once per CIGAR run).
+template <class K> DI int pen_m(const Ctx<K>& c) { return K::kLds ? rdl(c.upen, 0) : c.um; }
+template <class K> DI int pen_x(const Ctx<K>& c) { return K::kLds ? rdl(c.upen, 1) : c.ux; }
+template <class K> DI int pen_o(const Ctx<K>& c) { return K::kLds ? rdl(c.upen, 2) : c.uo; }
+template <class K> DI int pen_e(const Ctx<K>& c) { return K::kLds ? rdl(c.upen, 3) : c.ue; }
+
 template <class K>
 DI void out_flush(Ctx<K>& c) {
   if (c.pend_len <= 0) return;
@@ -1127,9 +1144,9 @@ DI void out_flush(Ctx<K>& c) {
     c.out_n++;
   }
   // cigar_score_gap_affine (cigar.c:277-303) over maximal runs
-  if (c.pend_op == kOpM) c.alg_score -= c.um * c.pend_len;
-  else if (c.pend_op == kOpX) c.alg_score -= c.ux * c.pend_len;
-  else c.alg_score -= c.uo + c.ue * c.pend_len;
+  if (c.pend_op == kOpM) c.alg_score -= pen_m(c) * c.pend_len;
+  else if (c.pend_op == kOpX) c.alg_score -= pen_x(c) * c.pend_len;
+  else c.alg_score -= pen_o(c) + pen_e(c) * c.pend_len;
   c.pend_len = 0;
   c.pend_op = -1;
 }
@@ -1152,7 +1169,7 @@ DI void rev_emit(Ctx<K>& c, RevRuns& rr, int op, int n) {
   if (op == rr.op) { rr.len += n; return; }
   if (rr.len > 0) {
     if (rr.n >= (K::kLds ? kLdsRcap : c.L.rcap)) { c.spill = 1; return; }
-    if (c.lane == 0) c.pv[c.L.runs_tmp + rr.n] = (int)(((uint32_t)rr.len << 4) | (uint32_t)rr.op);
+    if (c.lane == 0) c.pv[runs_off(c) + rr.n] = (int)(((uint32_t)rr.len << 4) | (uint32_t)rr.op);
     rr.n++;
   }
   rr.op = op;
@@ -1274,7 +1291,7 @@ DI int backtrace(Ctx<K>& c, const Dir& b, int score) {
     }
     rev_emit(c, rr, kOpD, v);
     rev_emit(c, rr, kOpI, h);
-  } else if (v != 0 || h != 0 || (score != 0 && c.um == 0)) {
+  } else if (v != 0 || h != 0 || (score != 0 && pen_m(c) == 0)) {
     return -1;
   }
   if (c.spill) return 0;
@@ -1283,13 +1300,13 @@ DI int backtrace(Ctx<K>& c, const Dir& b, int score) {
   out_append(c, rr.op, rr.len);
   if constexpr (K::kLds) {
     for (int i = rr.n - 1; i >= 0; --i) {
-      const uint32_t run = (uint32_t)uni(c.pv[c.L.runs_tmp + i]);
+      const uint32_t run = (uint32_t)uni(c.pv[runs_off(c) + i]);
       out_append(c, (int)(run & 15u), (int)(run >> 4));
     }
   } else {   // 64 runs per memory round trip
     for (int i0 = rr.n - 1; i0 >= 0; i0 -= kWave) {
       const int i = i0 - c.lane;
-      const int v = i >= 0 ? c.pv[c.L.runs_tmp + i] : 0;
+      const int v = i >= 0 ? c.pv[runs_off(c) + i] : 0;
       const int cnt = imin(kWave, i0 + 1);
       for (int j = 0; j < cnt; ++j) {
         const uint32_t run = (uint32_t)rdl(v, j);
@@ -1339,7 +1356,7 @@ struct Sub {
 template <class K>
 DI void push(Ctx<K>& c, int& sp, const Sub& s) {
   if (sp >= (K::kLds ? kLdsScap : c.L.scap)) { c.spill = 1; return; }
-  int* e = c.pv + c.L.stack + sp * kSubInts;
+  int* e = c.pv + stack_off(c) + sp * kSubInts;
   e[0] = s.pair; e[1] = s.pb; e[2] = s.pe; e[3] = s.tb; e[4] = s.te;
   e[5] = s.cb; e[6] = s.ce; e[7] = s.rem; e[8] = s.lvl;
   ++sp;
@@ -1347,7 +1364,7 @@ DI void push(Ctx<K>& c, int& sp, const Sub& s) {
 template <class K>
 DI Sub pop(Ctx<K>& c, int& sp) {
   --sp;
-  const int* e = c.pv + c.L.stack + sp * kSubInts;
+  const int* e = c.pv + stack_off(c) + sp * kSubInts;
   return Sub{uni(e[0]), uni(e[1]), uni(e[2]), uni(e[3]), uni(e[4]), uni(e[5]), uni(e[6]), uni(e[7]),
              uni(e[8])};
 }
@@ -1413,7 +1430,8 @@ DI int score_pair(Ctx<K>& c, int plen, int tlen, int* wf_score) {
   int s = bp.score;
   if (st == kStEndReached) s = (f.status == kStEndReached) ? f.status_score : r.status_score;
   *wf_score = s;
-  c.alg_score = (c.um == 0) ? -s : ((-c.um) * (plen + tlen) - s) / 2;
+  const int um = pen_m(c);
+  c.alg_score = (um == 0) ? -s : ((-um) * (plen + tlen) - s) / 2;
   return 0;
 }
 
@@ -1826,7 +1844,11 @@ void biwfa_persistent(KParams p) {
   c.L = p.L;
   c.x = p.x; c.o = p.o; c.e = p.e; c.scope = p.scope;
   c.max_steps = p.max_steps;
-  c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
+  if (K::kLds) {
+    c.upen = lane == 0 ? p.um : lane == 1 ? p.ux : lane == 2 ? p.uo : p.ue;
+  } else {
+    c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
+  }
   c.gmask = p.gmask;
   c.epoch = (uni(*(volatile const int32_t*)p.calls) << 3) | p.epoch;
   const int count = uni(*(volatile const int32_t*)p.count);

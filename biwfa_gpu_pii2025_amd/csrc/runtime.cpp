@@ -480,6 +480,16 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
   }
   b->runs_cap = runs_off[n];
   b->tiers = plan_tiers(a, b);
+  {
+    // the LDS kernel takes these layout values as compile-time constants
+    const bw::WsLayout& L0 = b->tiers[0].L;
+    if (L0.data != 0 || L0.hcap != bw::kLdsHcap || L0.rcap != bw::kLdsRcap || L0.scap != bw::kLdsScap ||
+        L0.runs_tmp != 0 || L0.stack != bw::kLdsRcap) {
+      set_error("LDS-tier layout does not match the kernel's constants");
+      bw_dbatch_free(b);
+      return BW_STATUS_INTERNAL;
+    }
+  }
   auto fail = [&](hipError_t e) {
     set_error(std::string("device allocation/copy failed: ") + hipGetErrorString(e));
     bw_dbatch_free(b);
