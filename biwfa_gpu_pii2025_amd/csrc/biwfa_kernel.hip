@@ -168,8 +168,9 @@ struct Ctx {
   // headers in registers: ring (lane = ring*scope + slot) and base case (lane = score)
   int rlo[3], rhi[3];
   int blo[3], bhi[3], bkb[3];
-  // presence masks (scalar): ring bit ring*32+slot, base-case bit score (LDS kernel)
-  uint64_t rpres[3], bpres[3];
+  // ring presence masks (scalar): bit ring*32+slot; base-case presence is a
+  // header lo other than kAbsent
+  uint64_t rpres[3];
   int gmask;      // scores s with (s & gmask) != 0 are null steps (even penalties)
   uint32_t* out;
   int out_cap, out_n, pend_op, pend_len, alg_score;
@@ -292,12 +293,14 @@ DI int ring_base(const Ctx<K>& c, int ring, int slot, int comp) {
   return ws_data(c) + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
 }
 
-// Presence bit of (ring slot | base score) for component `comp`: ring slots use
-// bit ring*32 + slot, base-case scores (LDS kernel) bit score.
+// Presence of (ring slot | base score) for component `comp`: ring slots use
+// bit ring*32 + slot of a scalar mask, base-case scores (LDS kernel) a header
+// lo other than kAbsent in lane `score` (no scalar mask: the LDS kernel runs
+// close to its scalar register limit).
 template <class K>
 DI bool is_present(const Ctx<K>& c, const Dir& d, int score, int slot, int comp) {
   if (d.ring >= 0) return (sel3u(c.rpres, comp) >> (d.ring * 32 + slot)) & 1ull;
-  return (sel3u(c.bpres, comp) >> score) & 1ull;
+  return rdl(sel3(c.blo, comp), score) != kAbsent;
 }
 
 // Component `comp` at `score` (ring slot `slot`), WFA2's
@@ -361,9 +364,8 @@ DI void set_hdr(Ctx<K>& c, const Dir& d, int score, int slot, int comp, int lo, 
       lane_set(c.rhi, comp, c.lane, j, hi);
     }
   } else if (K::kLds) {
-    bit_set(c.bpres, comp, score, on);
+    lane_set(c.blo, comp, c.lane, score, lo);   // kAbsent marks an absent component
     if (on) {
-      lane_set(c.blo, comp, c.lane, score, lo);
       lane_set(c.bhi, comp, c.lane, score, hi);
       lane_set(c.bkb, comp, c.lane, score, kb);
     }
@@ -495,7 +497,8 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
                __builtin_amdgcn_ds_bpermute(hb, c.bhi[CD]), cq);
     kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
-    pm = sel3u(c.bpres, cq);
+    pbit = 0;
+    pm = lq != kAbsent ? 1ull : 0ull;
   } else {
     // global workspace: base-case headers of the last 64 scores in register
     // lanes (lane score % 64, lo = kAbsent when absent; see set_hdr); a
@@ -1875,7 +1878,7 @@ void biwfa_persistent(KParams p) {
       for (int q = 0; q < 3; ++q) {
         c.rlo[q] = kAbsent; c.rhi[q] = 0;
         c.blo[q] = kAbsent; c.bhi[q] = 0; c.bkb[q] = 0;
-        c.rpres[q] = 0; c.bpres[q] = 0;
+        c.rpres[q] = 0;
       }
       c.out = p.runs + p.runs_off[pair];
       c.out_cap = (int)(p.runs_off[pair + 1] - p.runs_off[pair]);
