@@ -299,7 +299,10 @@ DI int ring_base(const Ctx<K>& c, int ring, int slot, int comp) {
 // close to its scalar register limit).
 template <class K>
 DI bool is_present(const Ctx<K>& c, const Dir& d, int score, int slot, int comp) {
-  if (d.ring >= 0) return (sel3u(c.rpres, comp) >> (d.ring * 32 + slot)) & 1ull;
+  if (d.ring >= 0) {
+    if (K::kLds) return rdl(sel3(c.rlo, comp), d.ring * c.scope + slot) != kAbsent;
+    return (sel3u(c.rpres, comp) >> (d.ring * 32 + slot)) & 1ull;
+  }
   return rdl(sel3(c.blo, comp), score) != kAbsent;
 }
 
@@ -357,11 +360,16 @@ template <class K>
 DI void set_hdr(Ctx<K>& c, const Dir& d, int score, int slot, int comp, int lo, int hi, int kb) {
   const bool on = lo != kAbsent;
   if (d.ring >= 0) {
-    bit_set(c.rpres, comp, d.ring * 32 + slot, on);
-    if (on) {
-      const int j = d.ring * c.scope + slot;
-      lane_set(c.rlo, comp, c.lane, j, lo);
-      lane_set(c.rhi, comp, c.lane, j, hi);
+    const int j = d.ring * c.scope + slot;
+    if (K::kLds) {
+      lane_set(c.rlo, comp, c.lane, j, lo);   // kAbsent marks an absent component
+      if (on) lane_set(c.rhi, comp, c.lane, j, hi);
+    } else {
+      bit_set(c.rpres, comp, d.ring * 32 + slot, on);
+      if (on) {
+        lane_set(c.rlo, comp, c.lane, j, lo);
+        lane_set(c.rhi, comp, c.lane, j, hi);
+      }
     }
   } else if (K::kLds) {
     lane_set(c.blo, comp, c.lane, score, lo);   // kAbsent marks an absent component
@@ -486,7 +494,12 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
     hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
     kq = ws_data(c) + (hl * 3 + cq) * c.L.wcap;
-    pm = sel3u(c.rpres, cq);
+    if (K::kLds) {
+      pbit = 0;
+      pm = lq != kAbsent ? 1ull : 0ull;
+    } else {
+      pm = sel3u(c.rpres, cq);
+    }
   } else if constexpr (K::kLds) {
     hl = sq;
     pbit = sq & (kWave - 1);
@@ -892,7 +905,6 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
   int sl = slot1 - i;
   sl += sl < 0 ? c.scope : 0;
   const int comp = cc == 0 ? CD : cc == 1 ? CI : CM;
-  const uint64_t pres = sel3u(c.rpres, comp);
   const int j = a1.ring * c.scope + sl;              // header lane of a1's slot
   const int jb = (j & (kWave - 1)) << 2;
   const int lo_m = __builtin_amdgcn_ds_bpermute(jb, c.rlo[CM]);
@@ -909,10 +921,10 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
   const int hi0 = pick3(rdl(c.rhi[CD], j0), rdl(c.rhi[CI], j0), rdl(c.rhi[CM], j0), cc);
   const int b0 = ws_data(c) + (j0 * 3 + comp) * c.L.wcap;
   const int b1 = ws_data(c) + (j * 3 + comp) * c.L.wcap;
-  const bool p0 = (pres >> (a0.ring * 32 + slot0)) & 1ull;
+  const bool p0 = lo0 != kAbsent, p1 = lo1 != kAbsent;   // presence: header lo set
   const int klo = imax(lo0, dk - hi1), khi = imin(hi0, dk - lo1);
-  const bool ok = i < c.scope && si >= 0 && (si & c.gmask) == 0 && sc < bp.score && p0 &&
-                  ((pres >> (a1.ring * 32 + sl)) & 1ull) && klo <= khi;
+  const bool ok = i < c.scope && si >= 0 && (si & c.gmask) == 0 && sc < bp.score && p0 && p1 &&
+                  klo <= khi;
   const uint64_t vm = __ballot(ok);
   if (!vm) return;
   const int nr = __builtin_popcountll(vm);
@@ -976,7 +988,7 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
   if constexpr (K::kLds) {
     if (3 * c.scope <= kWave && c.L.wcap <= kWave && 2 * 3 * c.scope * c.L.wcap <= 4096 &&
         6 * c.scope <= kLdsRcap && s0 + s1 < (1 << 18)) {
-      if (!((c.rpres[CM] >> (a0.ring * 32 + slot0)) & 1ull)) return;   // no M wavefront at s0
+      if (rdl(c.rlo[CM], a0.ring * c.scope + slot0) == kAbsent) return;   // no M wavefront at s0
       overlap_lanes(c, a0, slot0, a1, s0, s1, slot1, forward, tl, pl, bp);
       return;
     }
@@ -987,7 +999,8 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
   const Wf i0 = get_wf(c, a0, s0, slot0, CI, &pi0);
   const Wf d0 = get_wf(c, a0, s0, slot0, CD, &pd0);
   // ring slots of a1 holding any component (null scores hold none)
-  const uint64_t live1 = ((c.rpres[0] | c.rpres[1] | c.rpres[2]) >> (32 * a1.ring)) & 0xffffffffull;
+  const uint64_t live1 = K::kLds ? 0xffffffffull   // (LDS kernel: no masks; get_wf tests presence)
+                                 : ((c.rpres[0] | c.rpres[1] | c.rpres[2]) >> (32 * a1.ring)) & 0xffffffffull;
   if constexpr (K::kNW == 1) {
     for (int i = 0; i < c.scope; ++i) {
       const int si = s1 - i;
@@ -1060,8 +1073,13 @@ template <class K>
 DI void ring_init(Ctx<K>& c, Dir& d) {
   // a team's slower waves may still read the previous sub-problem's workspace
   if constexpr (K::kNW > 1) __syncthreads();
-  const uint64_t keep = ~(0xffffffffull << (32 * d.ring));
-  for (int q = 0; q < 3; ++q) c.rpres[q] &= keep;
+  if constexpr (K::kLds) {   // the ring's header lanes read absent
+    const bool mine = c.lane >= d.ring * c.scope && c.lane < (d.ring + 1) * c.scope;
+    for (int q = 0; q < 3; ++q) c.rlo[q] = mine ? kAbsent : c.rlo[q];
+  } else {
+    const uint64_t keep = ~(0xffffffffull << (32 * d.ring));
+    for (int q = 0; q < 3; ++q) c.rpres[q] &= keep;
+  }
   set_hdr(c, d, 0, 0, d.cb, 0, 0, 0);
   c.ws[ring_base(c, d.ring, 0, d.cb)] = 0;
   d.null_steps = 0;
