@@ -168,9 +168,7 @@ struct Ctx {
   // headers in registers: ring (lane = ring*scope + slot) and base case (lane = score)
   int rlo[3], rhi[3];
   int blo[3], bhi[3], bkb[3];
-  // ring presence masks (scalar): bit ring*32+slot; base-case presence is a
-  // header lo other than kAbsent
-  uint64_t rpres[3];
+  // presence: a header lo other than kAbsent (ring and base-case headers alike)
   int gmask;      // scores s with (s & gmask) != 0 are null steps (even penalties)
   uint32_t* out;
   int out_cap, out_n, pend_op, pend_len, alg_score;
@@ -194,10 +192,6 @@ DI int score_guard(const Ctx<K>& c, int plen, int tlen) {
 // folded into a select of their addresses and pin the aligner state to scratch.
 DI int sel3(const int (&a)[3], int comp) {
   return (a[0] & -(int)(comp == CM)) | (a[1] & -(int)(comp == CI)) | (a[2] & -(int)(comp == CD));
-}
-DI uint64_t sel3u(const uint64_t (&a)[3], int comp) {
-  return (a[0] & -(uint64_t)(comp == CM)) | (a[1] & -(uint64_t)(comp == CI)) |
-         (a[2] & -(uint64_t)(comp == CD));
 }
 
 // ring slot of (score - delta) given the slot of score; 0 <= delta < scope
@@ -293,16 +287,13 @@ DI int ring_base(const Ctx<K>& c, int ring, int slot, int comp) {
   return ws_data(c) + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
 }
 
-// Presence of (ring slot | base score) for component `comp`: ring slots use
-// bit ring*32 + slot of a scalar mask, base-case scores (LDS kernel) a header
-// lo other than kAbsent in lane `score` (no scalar mask: the LDS kernel runs
-// close to its scalar register limit).
+// Presence of (ring slot | base score) for component `comp`: a header lo other
+// than kAbsent in the slot's header lane (ring) or lane `score` (base case, LDS
+// kernel).  No scalar presence masks: the kernels run close to their scalar
+// register limit, and spilled scalars cost more than a lane read.
 template <class K>
 DI bool is_present(const Ctx<K>& c, const Dir& d, int score, int slot, int comp) {
-  if (d.ring >= 0) {
-    if (K::kLds) return rdl(sel3(c.rlo, comp), d.ring * c.scope + slot) != kAbsent;
-    return (sel3u(c.rpres, comp) >> (d.ring * 32 + slot)) & 1ull;
-  }
+  if (d.ring >= 0) return rdl(sel3(c.rlo, comp), d.ring * c.scope + slot) != kAbsent;
   return rdl(sel3(c.blo, comp), score) != kAbsent;
 }
 
@@ -347,13 +338,6 @@ DI void lane_set(int (&a)[3], int comp, int lane, int j, int v) {
   a[2] = (me && comp == CD) ? v : a[2];
 }
 
-DI void bit_set(uint64_t (&m)[3], int comp, int bit, bool on) {
-  const uint64_t b = 1ull << bit;
-  const uint64_t set = on ? b : 0ull;
-  m[0] = comp == CM ? ((m[0] & ~b) | set) : m[0];
-  m[1] = comp == CI ? ((m[1] & ~b) | set) : m[1];
-  m[2] = comp == CD ? ((m[2] & ~b) | set) : m[2];
-}
 
 // lo == kAbsent marks an unallocated component
 template <class K>
@@ -361,16 +345,8 @@ DI void set_hdr(Ctx<K>& c, const Dir& d, int score, int slot, int comp, int lo, 
   const bool on = lo != kAbsent;
   if (d.ring >= 0) {
     const int j = d.ring * c.scope + slot;
-    if (K::kLds) {
-      lane_set(c.rlo, comp, c.lane, j, lo);   // kAbsent marks an absent component
-      if (on) lane_set(c.rhi, comp, c.lane, j, hi);
-    } else {
-      bit_set(c.rpres, comp, d.ring * 32 + slot, on);
-      if (on) {
-        lane_set(c.rlo, comp, c.lane, j, lo);
-        lane_set(c.rhi, comp, c.lane, j, hi);
-      }
-    }
+    lane_set(c.rlo, comp, c.lane, j, lo);   // kAbsent marks an absent component
+    if (on) lane_set(c.rhi, comp, c.lane, j, hi);
   } else if (K::kLds) {
     lane_set(c.blo, comp, c.lane, score, lo);   // kAbsent marks an absent component
     if (on) {
@@ -494,12 +470,8 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
     hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
     kq = ws_data(c) + (hl * 3 + cq) * c.L.wcap;
-    if (K::kLds) {
-      pbit = 0;
-      pm = lq != kAbsent ? 1ull : 0ull;
-    } else {
-      pm = sel3u(c.rpres, cq);
-    }
+    pbit = 0;
+    pm = lq != kAbsent ? 1ull : 0ull;
   } else if constexpr (K::kLds) {
     hl = sq;
     pbit = sq & (kWave - 1);
@@ -999,8 +971,7 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
   const Wf i0 = get_wf(c, a0, s0, slot0, CI, &pi0);
   const Wf d0 = get_wf(c, a0, s0, slot0, CD, &pd0);
   // ring slots of a1 holding any component (null scores hold none)
-  const uint64_t live1 = K::kLds ? 0xffffffffull   // (LDS kernel: no masks; get_wf tests presence)
-                                 : ((c.rpres[0] | c.rpres[1] | c.rpres[2]) >> (32 * a1.ring)) & 0xffffffffull;
+  const uint64_t live1 = 0xffffffffull;   // every slot is examined; get_wf tests presence
   if constexpr (K::kNW == 1) {
     for (int i = 0; i < c.scope; ++i) {
       const int si = s1 - i;
@@ -1073,12 +1044,9 @@ template <class K>
 DI void ring_init(Ctx<K>& c, Dir& d) {
   // a team's slower waves may still read the previous sub-problem's workspace
   if constexpr (K::kNW > 1) __syncthreads();
-  if constexpr (K::kLds) {   // the ring's header lanes read absent
+  {   // the ring's header lanes read absent
     const bool mine = c.lane >= d.ring * c.scope && c.lane < (d.ring + 1) * c.scope;
     for (int q = 0; q < 3; ++q) c.rlo[q] = mine ? kAbsent : c.rlo[q];
-  } else {
-    const uint64_t keep = ~(0xffffffffull << (32 * d.ring));
-    for (int q = 0; q < 3; ++q) c.rpres[q] &= keep;
   }
   set_hdr(c, d, 0, 0, d.cb, 0, 0, 0);
   c.ws[ring_base(c, d.ring, 0, d.cb)] = 0;
@@ -1896,7 +1864,6 @@ void biwfa_persistent(KParams p) {
       for (int q = 0; q < 3; ++q) {
         c.rlo[q] = kAbsent; c.rhi[q] = 0;
         c.blo[q] = kAbsent; c.bhi[q] = 0; c.bkb[q] = 0;
-        c.rpres[q] = 0;
       }
       c.out = p.runs + p.runs_off[pair];
       c.out_cap = (int)(p.runs_off[pair + 1] - p.runs_off[pair]);
