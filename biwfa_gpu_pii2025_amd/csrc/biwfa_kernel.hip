@@ -457,24 +457,19 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
   const int cq = q <= 1 ? CM : q == 2 ? CI : CD;
   const int sq = s - delta;
   const bool ring = d.ring >= 0;
-  int hl, pbit, lq, hq, kq;
-  uint64_t pm;
+  int hl, lq, hq, kq;
   if (ring) {
     int sl = slot - delta;
     sl += sl < 0 ? c.scope : 0;
     hl = d.ring * c.scope + sl;
-    pbit = d.ring * 32 + sl;
     const int hb = (hl & (kWave - 1)) << 2;
     lq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rlo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rlo[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rlo[CD]), cq);
     hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
     kq = ws_data(c) + (hl * 3 + cq) * c.L.wcap;
-    pbit = 0;
-    pm = lq != kAbsent ? 1ull : 0ull;
   } else if constexpr (K::kLds) {
     hl = sq;
-    pbit = sq & (kWave - 1);
     const int hb = (hl & (kWave - 1)) << 2;
     lq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.blo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.blo[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.blo[CD]), cq);
@@ -482,8 +477,6 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
                __builtin_amdgcn_ds_bpermute(hb, c.bhi[CD]), cq);
     kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
-    pbit = 0;
-    pm = lq != kAbsent ? 1ull : 0ull;
   } else {
     // global workspace: base-case headers of the last 64 scores in register
     // lanes (lane score % 64, lo = kAbsent when absent; see set_hdr); a
@@ -497,10 +490,8 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
     kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
-    pbit = 0;
-    pm = lq != kAbsent ? 1ull : 0ull;
   }
-  const bool present = q < 4 && sq >= 0 && ((pm >> pbit) & 1ull);
+  const bool present = q < 4 && sq >= 0 && lq != kAbsent;   // a header lo other than kAbsent
   const bool nonempty = present && lq <= hq;
   if (!nonempty) {
     lq = 1;
