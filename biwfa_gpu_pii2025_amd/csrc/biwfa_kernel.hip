@@ -131,7 +131,7 @@ DI int rd(const int32_t* ws, const Wf& w, int k) {
 DI int rd_clamped(const int32_t* ws, const Wf& w, int k) {
   const int kk = imin(imax(k, w.lo), w.hi);
   const int v = ws[(uint32_t)(w.base + (kk & w.mask))];
-  return (k >= w.lo && k <= w.hi) ? v : kOffsetNull;
+  return ((k >= w.lo) & (k <= w.hi)) ? v : kOffsetNull;
 }
 
 // One aligner's view of a sub-problem.
@@ -384,40 +384,51 @@ DI uint64_t window2(uint64_t lo, uint64_t hi, int sh) {
 typedef const __attribute__((address_space(3))) uint64_t* lds_words;
 typedef const __attribute__((address_space(1))) uint64_t* global_words;
 
-template <class Ptr>
+template <bool kOneExit, class Ptr>
 DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h);
 
 // Extension with the sequence loads in the address space the pair's words
 // live in (generic loads would wait on both the LDS and the memory counters).
 template <class K>
 DI int lcp(const Ctx<K>& c, const Dir& d, int v, int h) {
-  if (K::kLds || c.seq_lds) return lcp_words((lds_words)c.P, (lds_words)c.T, d, v, h);
-  return lcp_words((global_words)c.P, (global_words)c.T, d, v, h);
+  if (K::kLds || c.seq_lds) return lcp_words<K::kLds>((lds_words)c.P, (lds_words)c.T, d, v, h);
+  return lcp_words<false>((global_words)c.P, (global_words)c.T, d, v, h);
 }
 
-template <class Ptr>
+template <bool kOneExit, class Ptr>
 DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h) {
   const int lim = imin(d.plen - v, d.tlen - h);
   // forward: windows start at P[pb+v+n], T[tb+h+n]; reverse: windows end at
   // P[pe-1-v-n], T[te-1-h-n] and the first mismatch is the highest differing base
   const bool rev = d.rev != 0;
-  int pp = rev ? d.pb + d.plen - 32 - v : d.pb + v;
-  int tp = rev ? d.tb + d.tlen - 32 - h : d.tb + h;
-  const int step = rev ? -32 : 32;
+  const int pp = rev ? d.pb + d.plen - 32 - v : d.pb + v;
+  const int tp = rev ? d.tb + d.tlen - 32 - h : d.tb + h;
+  // windows move 32 bases (one word) per iteration: the in-word shifts stay put
+  const int sp = pp & 31, st = tp & 31, dw = rev ? -1 : 1;
+  int wp = pp >> 5, wt = tp >> 5;
   int n = 0;
   for (;;) {
-    const int wp = pp >> 5, wt = tp >> 5;
     const uint64_t p0 = P[wp], p1 = P[wp + 1];
     const uint64_t t0 = T[wt], t1 = T[wt + 1];
-    const uint64_t x = window2(p0, p1, pp & 31) ^ window2(t0, t1, tp & 31);
-    if (x) {
-      n += (rev ? __builtin_clzll(x) : __builtin_ctzll(x)) >> 1;
-      break;
+    const uint64_t x = window2(p0, p1, sp) ^ window2(t0, t1, st);
+    if constexpr (kOneExit) {
+      // One exit test per word: a lane leaves at its first mismatch or past
+      // the limit (a mismatch-free word counts 32 bases), so the divergent
+      // loop carries a single break mask instead of one per exit (fewer scalar
+      // mask operations per word, in the kernel that is bound by issue).
+      const int z = x == 0 ? 64 : rev ? __builtin_clzll(x) : __builtin_ctzll(x);
+      n += z >> 1;
+      if (x != 0 || n >= lim) break;
+    } else {
+      if (x) {
+        n += (rev ? __builtin_clzll(x) : __builtin_ctzll(x)) >> 1;
+        break;
+      }
+      n += 32;
+      if (n >= lim) break;
     }
-    n += 32;
-    if (n >= lim) break;
-    pp += step;
-    tp += step;
+    wp += dw;
+    wt += dw;
   }
   return imin(n, lim);
 }
