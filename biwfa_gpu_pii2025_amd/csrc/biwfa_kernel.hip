@@ -133,6 +133,12 @@ DI int rd_clamped(const int32_t* ws, const Wf& w, int k) {
   const int v = ws[(uint32_t)(w.base + (kk & w.mask))];
   return ((k >= w.lo) & (k <= w.hi)) ? v : kOffsetNull;
 }
+// rd without a branch for a ring input: base + (k & mask) stays inside the
+// ring slot for every k, so no clamp is needed.
+DI int rd_ring(const int32_t* ws, const Wf& w, int k) {
+  const int v = ws[(uint32_t)(w.base + (k & w.mask))];
+  return ((k >= w.lo) & (k <= w.hi)) ? v : kOffsetNull;
+}
 
 // One aligner's view of a sub-problem.
 struct Dir {
@@ -597,9 +603,16 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     // no loop, branch-free input reads, trims straight from the ballots.
     const int k = lo + c.lane;
     const bool act = k <= hi;
-    const int ins1 = imax(rd_clamped(ws, mo, k - 1), rd_clamped(ws, ie, k - 1)) + 1;
-    const int del1 = imax(rd_clamped(ws, mo, k + 1), rd_clamped(ws, de, k + 1));
-    int mx = imax(del1, imax(rd_clamped(ws, mm, k) + 1, ins1));
+    int ins1, del1, mx;
+    if (d.ring >= 0) {
+      ins1 = imax(rd_ring(ws, mo, k - 1), rd_ring(ws, ie, k - 1)) + 1;
+      del1 = imax(rd_ring(ws, mo, k + 1), rd_ring(ws, de, k + 1));
+      mx = imax(del1, imax(rd_ring(ws, mm, k) + 1, ins1));
+    } else {
+      ins1 = imax(rd_clamped(ws, mo, k - 1), rd_clamped(ws, ie, k - 1)) + 1;
+      del1 = imax(rd_clamped(ws, mo, k + 1), rd_clamped(ws, de, k + 1));
+      mx = imax(del1, imax(rd_clamped(ws, mm, k) + 1, ins1));
+    }
     const bool live = act && !((uint32_t)mx > tl || (uint32_t)(mx - k) > pl);
     int ak = 0;
     if (live) {                                   // extension along the diagonal
