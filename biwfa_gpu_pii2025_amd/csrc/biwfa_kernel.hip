@@ -170,6 +170,8 @@ struct Ctx {
   int x, o, e, scope, max_steps;
   int um, ux, uo, ue;   // user penalties (global kernels)
   int upen;             // the same in lanes 0..3 (LDS kernel)
+  int qm0, qm1, qm2;    // LDS kernel: lane masks (-1 / 0) of the input gather's component
+  int qlo, qhi;         // LDS kernel: the gather's per-lane limit adjustments
   int arena_top;
   // headers in registers: ring (lane = ring*scope + slot) and base case (lane = score)
   int rlo[3], rhi[3];
@@ -466,6 +468,13 @@ DI void null_step(Ctx<K>& c, Dir& d, int s, int slot) {
 // Returns false when all four inputs are null.  Ring inputs use mask wcap-1,
 // base-case inputs mask -1; a null input's base keeps a clamped read inside the
 // workspace.
+// pick3 by the gather's per-lane component (CM, CI, CD for lanes 0-1, 2, 3)
+template <class K>
+DI int pick_q(const Ctx<K>& c, int v0, int v1, int v2, int cq) {
+  if constexpr (K::kLds) return (v0 & c.qm0) | (v1 & c.qm1) | (v2 & c.qm2);
+  return pick3(v0, v1, v2, cq);
+}
+
 template <class K>
 DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf& mo, Wf& ie, Wf& de,
                       int& lo, int& hi, bool& has_i, bool& has_d) {
@@ -480,19 +489,19 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
     sl += sl < 0 ? c.scope : 0;
     hl = d.ring * c.scope + sl;
     const int hb = (hl & (kWave - 1)) << 2;
-    lq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rlo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rlo[CI]),
+    lq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.rlo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rlo[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rlo[CD]), cq);
-    hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
+    hq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
     kq = ws_data(c) + (hl * 3 + cq) * c.L.wcap;
   } else if constexpr (K::kLds) {
     hl = sq;
     const int hb = (hl & (kWave - 1)) << 2;
-    lq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.blo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.blo[CI]),
+    lq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.blo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.blo[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.blo[CD]), cq);
-    hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bhi[CI]),
+    hq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.bhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bhi[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.bhi[CD]), cq);
-    kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
+    kq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
   } else {
     // global workspace: base-case headers of the last 64 scores in register
@@ -501,14 +510,16 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
     // between was written
     hl = sq;
     const int hb = (hl & (kWave - 1)) << 2;
-    lq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rlo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rlo[CI]),
+    lq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.rlo[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rlo[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rlo[CD]), cq);
-    hq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
+    hq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
-    kq = pick3(__builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
+    kq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.bkb[CM]), __builtin_amdgcn_ds_bpermute(hb, c.bkb[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.bkb[CD]), cq);
   }
-  const bool present = q < 4 && sq >= 0 && lq != kAbsent;   // a header lo other than kAbsent
+  // Only lanes 0..3 are read below (their readlanes, the ballot's low 4 bits
+  // and the limits reduced into lane 3), so lanes 4.. need no masking.
+  const bool present = sq >= 0 && lq != kAbsent;   // a header lo other than kAbsent
   const bool nonempty = present && lq <= hq;
   if (!nonempty) {
     lq = 1;
@@ -516,9 +527,14 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
     kq = ring ? 0 : 1;
   }
   // limits: M[s-x] [lo, hi], M[s-o-e] [lo-1, hi+1], I1[s-e] [lo+1, hi+1], D1[s-e] [lo-1, hi-1]
-  int lc = lq + (q == 0 ? 0 : q == 2 ? 1 : -1);
-  int hc = hq + (q == 0 ? 0 : q == 3 ? -1 : 1);
-  if (q >= 4) { lc = INT_MAX; hc = INT_MIN; }
+  int lc, hc;
+  if constexpr (K::kLds) {
+    lc = lq + c.qlo;
+    hc = hq + c.qhi;
+  } else {
+    lc = lq + (q == 0 ? 0 : q == 2 ? 1 : -1);
+    hc = hq + (q == 0 ? 0 : q == 3 ? -1 : 1);
+  }
   lc = imin(lc, __builtin_amdgcn_update_dpp(INT_MAX, lc, 0x111, 0xf, 0xf, false));  // row_shr:1
   hc = imax(hc, __builtin_amdgcn_update_dpp(INT_MIN, hc, 0x111, 0xf, 0xf, false));
   lc = imin(lc, __builtin_amdgcn_update_dpp(INT_MAX, lc, 0x112, 0xf, 0xf, false));  // row_shr:2
@@ -1850,6 +1866,14 @@ void biwfa_persistent(KParams p) {
   c.max_steps = p.max_steps;
   if (K::kLds) {
     c.upen = lane == 0 ? p.um : lane == 1 ? p.ux : lane == 2 ? p.uo : p.ue;
+    // Kept as vector values: as compares they would become scalar lane masks
+    // hoisted out of every loop, and the kernel has no scalar registers to spare.
+    const int cq = lane <= 1 ? CM : lane == 2 ? CI : CD;
+    int m0 = -(int)(cq == CM), m1 = -(int)(cq == CI), m2 = -(int)(cq == CD);
+    int al = lane == 0 ? 0 : lane == 2 ? 1 : -1, ah = lane == 0 ? 0 : lane == 3 ? -1 : 1;
+    asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(al), "+v"(ah));
+    c.qm0 = m0; c.qm1 = m1; c.qm2 = m2;
+    c.qlo = al; c.qhi = ah;
   } else {
     c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
   }
