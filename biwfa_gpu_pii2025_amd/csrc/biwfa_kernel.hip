@@ -172,6 +172,8 @@ struct Ctx {
   int upen;             // the same in lanes 0..3 (LDS kernel)
   int qm0, qm1, qm2;    // LDS kernel: lane masks (-1 / 0) of the input gather's component
   int qlo, qhi;         // LDS kernel: the gather's per-lane limit adjustments
+  int qdelta, qcomp;    // LDS kernel: the gather's per-lane score distance and component
+  int om0, om1, om2;    // LDS kernel: lane masks of the overlap candidates' kind (p % 3)
   int arena_top;
   // headers in registers: ring (lane = ring*scope + slot) and base case (lane = score)
   int rlo[3], rhi[3];
@@ -479,8 +481,9 @@ template <class K>
 DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf& mo, Wf& ie, Wf& de,
                       int& lo, int& hi, bool& has_i, bool& has_d) {
   const int q = c.lane;
-  const int delta = pick3(c.x, c.o + c.e, c.e, imin(q, 2));   // q = 3 shares s-e with q = 2
-  const int cq = q <= 1 ? CM : q == 2 ? CI : CD;
+  // q = 3 shares s-e with q = 2
+  const int delta = K::kLds ? c.qdelta : pick3(c.x, c.o + c.e, c.e, imin(q, 2));
+  const int cq = K::kLds ? c.qcomp : q <= 1 ? CM : q == 2 ? CI : CD;
   const int sq = s - delta;
   const bool ring = d.ring >= 0;
   int hl, lq, hq, kq;
@@ -904,10 +907,10 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
   const int i = lane_div3(p);
   const int cc = p - 3 * i;                          // 0: D, 1: I, 2: M
   const int si = s1 - i;
-  const int sc = s0 + si - (cc == 2 ? 0 : c.o);
+  const int sc = s0 + si - (c.o & ~c.om2);          // an indel candidate pays the opening
   int sl = slot1 - i;
   sl += sl < 0 ? c.scope : 0;
-  const int comp = cc == 0 ? CD : cc == 1 ? CI : CM;
+  const int comp = 2 - cc;                           // CD, CI, CM
   const int j = a1.ring * c.scope + sl;              // header lane of a1's slot
   const int jb = (j & (kWave - 1)) << 2;
   const int lo_m = __builtin_amdgcn_ds_bpermute(jb, c.rlo[CM]);
@@ -916,12 +919,12 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
   const int hi_m = __builtin_amdgcn_ds_bpermute(jb, c.rhi[CM]);
   const int hi_i = __builtin_amdgcn_ds_bpermute(jb, c.rhi[CI]);
   const int hi_d = __builtin_amdgcn_ds_bpermute(jb, c.rhi[CD]);
-  const int lo1 = pick3(lo_d, lo_i, lo_m, cc);
-  const int hi1 = pick3(hi_d, hi_i, hi_m, cc);
+  const int lo1 = (lo_d & c.om0) | (lo_i & c.om1) | (lo_m & c.om2);
+  const int hi1 = (hi_d & c.om0) | (hi_i & c.om1) | (hi_m & c.om2);
   // a0's component of this lane's candidate, at score s0 (header lane j0)
   const int j0 = a0.ring * c.scope + slot0;
-  const int lo0 = pick3(rdl(c.rlo[CD], j0), rdl(c.rlo[CI], j0), rdl(c.rlo[CM], j0), cc);
-  const int hi0 = pick3(rdl(c.rhi[CD], j0), rdl(c.rhi[CI], j0), rdl(c.rhi[CM], j0), cc);
+  const int lo0 = (rdl(c.rlo[CD], j0) & c.om0) | (rdl(c.rlo[CI], j0) & c.om1) | (rdl(c.rlo[CM], j0) & c.om2);
+  const int hi0 = (rdl(c.rhi[CD], j0) & c.om0) | (rdl(c.rhi[CI], j0) & c.om1) | (rdl(c.rhi[CM], j0) & c.om2);
   const int b0 = ws_data(c) + (j0 * 3 + comp) * c.L.wcap;
   const int b1 = ws_data(c) + (j * 3 + comp) * c.L.wcap;
   const bool p0 = lo0 != kAbsent, p1 = lo1 != kAbsent;   // presence: header lo set
@@ -1871,9 +1874,15 @@ void biwfa_persistent(KParams p) {
     const int cq = lane <= 1 ? CM : lane == 2 ? CI : CD;
     int m0 = -(int)(cq == CM), m1 = -(int)(cq == CI), m2 = -(int)(cq == CD);
     int al = lane == 0 ? 0 : lane == 2 ? 1 : -1, ah = lane == 0 ? 0 : lane == 3 ? -1 : 1;
+    int qd = lane == 0 ? p.x : lane == 1 ? p.o + p.e : p.e, qc = cq;
+    const int oc = lane % 3;
+    int o0 = -(int)(oc == 0), o1 = -(int)(oc == 1), o2 = -(int)(oc == 2);
     asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(al), "+v"(ah));
+    asm volatile("" : "+v"(qd), "+v"(qc), "+v"(o0), "+v"(o1), "+v"(o2));
     c.qm0 = m0; c.qm1 = m1; c.qm2 = m2;
     c.qlo = al; c.qhi = ah;
+    c.qdelta = qd; c.qcomp = qc;
+    c.om0 = o0; c.om1 = o1; c.om2 = o2;
   } else {
     c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
   }
