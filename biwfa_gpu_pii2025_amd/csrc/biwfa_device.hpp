@@ -35,12 +35,16 @@ constexpr int kQctlInts = 6;           // per tier: {queue, count, head, tail, i
 
 // Kernel configuration: kLds keeps the whole workspace and both sequences in
 // LDS (one wave per pair); otherwise the workspace is a global slab shared by a
-// team of kNW waves (kNW > 1: the team is one workgroup).
-template <bool L, int NW>
+// team of kNW waves (kNW > 1: the team is one workgroup).  kW > 0 fixes the
+// ring width at compile time (the LDS tier's default width); 0 reads it from
+// the layout.
+template <bool L, int NW, int W = 0>
 struct Cfg {
   static constexpr bool kLds = L;
   static constexpr int kNW = NW;
+  static constexpr int kW = W;
 };
+constexpr int kLdsDefaultWidth = 32;   // LDS-tier ring width unless tuned otherwise
 
 // Workspace layout of one aligner (a wave, or a team of waves), in int32 units
 // from its workspace base.  The ring / arena is shared by the team; every wave

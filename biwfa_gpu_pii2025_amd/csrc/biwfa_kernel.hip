@@ -283,6 +283,9 @@ DI int team_min_slot(const Ctx<K>& c, const int32_t* buf, int j) {
 // scalar register limit).
 template <class K>
 DI int ws_data(const Ctx<K>& c) { return K::kLds ? 0 : c.L.data; }
+// ring width (a power of two): compile-time in the default LDS configuration
+template <class K>
+DI int wcap(const Ctx<K>& c) { return K::kW > 0 ? K::kW : c.L.wcap; }
 
 // Offsets in a wave's private region of the backtrace run buffer and the
 // recursion stack (compile-time in the LDS kernel: its region holds no
@@ -294,7 +297,7 @@ DI int stack_off(const Ctx<K>& c) { return K::kLds ? kLdsRcap : c.L.stack; }
 
 template <class K>
 DI int ring_base(const Ctx<K>& c, int ring, int slot, int comp) {
-  return ws_data(c) + ((ring * c.scope + slot) * 3 + comp) * c.L.wcap;
+  return ws_data(c) + ((ring * c.scope + slot) * 3 + comp) * wcap(c);
 }
 
 // Presence of (ring slot | base score) for component `comp`: a header lo other
@@ -321,7 +324,7 @@ DI Wf get_wf(const Ctx<K>& c, const Dir& d, int score, int slot, int comp, int* 
     const int lo = rdl(sel3(c.rlo, comp), j);
     const int hi = rdl(sel3(c.rhi, comp), j);
     if (lo > hi) return Wf{lo, hi, 0, 0};
-    return Wf{lo, hi, ring_base(c, d.ring, slot, comp), c.L.wcap - 1};
+    return Wf{lo, hi, ring_base(c, d.ring, slot, comp), wcap(c) - 1};
   }
   int lo, hi, kb;
   if (K::kLds) {
@@ -496,7 +499,7 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
                __builtin_amdgcn_ds_bpermute(hb, c.rlo[CD]), cq);
     hq = pick_q(c, __builtin_amdgcn_ds_bpermute(hb, c.rhi[CM]), __builtin_amdgcn_ds_bpermute(hb, c.rhi[CI]),
                __builtin_amdgcn_ds_bpermute(hb, c.rhi[CD]), cq);
-    kq = ws_data(c) + (hl * 3 + cq) * c.L.wcap;
+    kq = ws_data(c) + (hl * 3 + cq) * wcap(c);
   } else if constexpr (K::kLds) {
     hl = sq;
     const int hb = (hl & (kWave - 1)) << 2;
@@ -543,7 +546,7 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
   lc = imin(lc, __builtin_amdgcn_update_dpp(INT_MAX, lc, 0x112, 0xf, 0xf, false));  // row_shr:2
   hc = imax(hc, __builtin_amdgcn_update_dpp(INT_MIN, hc, 0x112, 0xf, 0xf, false));
   const uint32_t ne = (uint32_t)__ballot(nonempty) & 0xfu;
-  const int mk = ring ? c.L.wcap - 1 : -1;
+  const int mk = ring ? wcap(c) - 1 : -1;
   mm = Wf{rdl(lq, 0), rdl(hq, 0), rdl(kq, 0), mk};
   mo = Wf{rdl(lq, 1), rdl(hq, 1), rdl(kq, 1), mk};
   ie = Wf{rdl(lq, 2), rdl(hq, 2), rdl(kq, 2), mk};
@@ -596,11 +599,11 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   const int width = hi - lo + 1;
   int bm, bi, bd, mask;
   if (d.ring >= 0) {
-    if (width > c.L.wcap) { c.spill = 1; return 0; }
+    if (width > wcap(c)) { c.spill = 1; return 0; }
     bm = ring_base(c, d.ring, slot, CM);
     bi = ring_base(c, d.ring, slot, CI);
     bd = ring_base(c, d.ring, slot, CD);
-    mask = c.L.wcap - 1;
+    mask = wcap(c) - 1;
   } else {
     const int need = width * (1 + (int)has_i + (int)has_d);
     if (c.arena_top + need > c.L.arena_cap) { c.spill = 1; return 0; }
@@ -925,8 +928,8 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
   const int j0 = a0.ring * c.scope + slot0;
   const int lo0 = (rdl(c.rlo[CD], j0) & c.om0) | (rdl(c.rlo[CI], j0) & c.om1) | (rdl(c.rlo[CM], j0) & c.om2);
   const int hi0 = (rdl(c.rhi[CD], j0) & c.om0) | (rdl(c.rhi[CI], j0) & c.om1) | (rdl(c.rhi[CM], j0) & c.om2);
-  const int b0 = ws_data(c) + (j0 * 3 + comp) * c.L.wcap;
-  const int b1 = ws_data(c) + (j * 3 + comp) * c.L.wcap;
+  const int b0 = ws_data(c) + (j0 * 3 + comp) * wcap(c);
+  const int b1 = ws_data(c) + (j * 3 + comp) * wcap(c);
   const bool p0 = lo0 != kAbsent, p1 = lo1 != kAbsent;   // presence: header lo set
   const int klo = imax(lo0, dk - hi1), khi = imin(hi0, dk - lo1);
   const bool ok = i < c.scope && si >= 0 && (si & c.gmask) == 0 && sc < bp.score && p0 && p1 &&
@@ -941,8 +944,8 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
     tab[2 * r + 1] = b0 | (b1 << 12) | (p << 24);
   }
   wave_sync<K::kLds>();
-  const int wm = c.L.wcap - 1;
-  const int sh = __builtin_ctz(c.L.wcap);
+  const int wm = wcap(c) - 1;
+  const int sh = __builtin_ctz(wcap(c));
   const int off = c.lane & wm;
   int best = INT_MAX;
   for (int r0 = 0; r0 < nr; r0 += kWave >> sh) {
@@ -992,7 +995,7 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
   if (s0 & c.gmask) return;     // a score that is not a multiple of g holds nothing
   const int tl = a0.tlen, pl = a0.plen;
   if constexpr (K::kLds) {
-    if (3 * c.scope <= kWave && c.L.wcap <= kWave && 2 * 3 * c.scope * c.L.wcap <= 4096 &&
+    if (3 * c.scope <= kWave && wcap(c) <= kWave && 2 * 3 * c.scope * wcap(c) <= 4096 &&
         6 * c.scope <= kLdsRcap && s0 + s1 < (1 << 18)) {
       if (rdl(c.rlo[CM], a0.ring * c.scope + slot0) == kAbsent) return;   // no M wavefront at s0
       overlap_lanes(c, a0, slot0, a1, s0, s1, slot1, forward, tl, pl, bp);
@@ -1985,6 +1988,7 @@ __global__ __launch_bounds__(256) void biwfa_compact(const uint32_t* runs, const
 }
 
 template __global__ void biwfa_persistent<Cfg<true, 1>>(KParams);
+template __global__ void biwfa_persistent<Cfg<true, 1, kLdsDefaultWidth>>(KParams);
 template __global__ void biwfa_persistent<Cfg<false, 1>>(KParams);
 template __global__ void biwfa_persistent<Cfg<false, 2>>(KParams);
 template __global__ void biwfa_persistent<Cfg<false, 4>>(KParams);

@@ -362,13 +362,16 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
     t.lds = true;
     t.seq_cap = tu.lds_seq_cap > 0 ? tu.lds_seq_cap : 512;
     const int seq_words = (t.seq_cap + 31) / 32 + 3;
-    const int lds_wcap = tu.lds_ring_width > 0 ? tu.lds_ring_width : 32;
+    const int lds_wcap = tu.lds_ring_width > 0 ? tu.lds_ring_width : bw::kLdsDefaultWidth;
     t.L = make_layout(scope, lds_wcap, 0, bw::kLdsHcap, bw::kLdsRcap, bw::kLdsScap, seq_words, false, 1);
     t.waves_per_block = bw::kWavesPerBlock;
     t.dyn_lds = (size_t)t.L.total * 4 * t.waves_per_block;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bw::biwfa_persistent<bw::Cfg<true, 1>>,
-                                                     bw::kWave * t.waves_per_block, t.dyn_lds) != hipSuccess ||
+    const void* lds_kernel = lds_wcap == bw::kLdsDefaultWidth
+                                 ? reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<true, 1, bw::kLdsDefaultWidth>>)
+                                 : reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<true, 1>>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lds_kernel, bw::kWave * t.waves_per_block, t.dyn_lds) !=
+            hipSuccess ||
         nb <= 0)
       nb = std::max<int>(1, (int)((160 * 1024) / t.dyn_lds));
     if (a->attr.waves_per_cu > 0) nb = std::max(1, std::min(nb, a->attr.waves_per_cu / t.waves_per_block));
@@ -654,7 +657,11 @@ static int launch_sequence(bw_aligner_t* a, bw_dbatch_t* b, bool events) {
     p.gws_stride = T.stride;
     p.seq_lds_words = T.seq_lds_words;
     if (T.lds) {
-      bw::biwfa_persistent<bw::Cfg<true, 1>><<<dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), T.dyn_lds, s>>>(p);
+      if (T.L.wcap == bw::kLdsDefaultWidth)   // the ring width as a compile-time constant
+        bw::biwfa_persistent<bw::Cfg<true, 1, bw::kLdsDefaultWidth>>
+            <<<dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), T.dyn_lds, s>>>(p);
+      else
+        bw::biwfa_persistent<bw::Cfg<true, 1>><<<dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), T.dyn_lds, s>>>(p);
       HIP_TRY(hipGetLastError());
     } else {
       void* args[] = {&p};
