@@ -308,7 +308,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
             "traffic": traffic_bytes(args.workload, n),
-            "kernel": "biwfa_persistent<Cfg<true,1,32>> (LDS tier, one wave per pair, ring width 32)",
+            "kernel": "biwfa_persistent<Cfg<true,1,32,true>> (LDS tier, one wave per pair, ring width 32, default penalties)",
             "algorithmic_bytes_per_launch": alg_bytes,
         },
     }

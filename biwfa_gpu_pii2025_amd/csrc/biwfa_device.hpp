@@ -37,14 +37,19 @@ constexpr int kQctlInts = 6;           // per tier: {queue, count, head, tail, i
 // LDS (one wave per pair); otherwise the workspace is a global slab shared by a
 // team of kNW waves (kNW > 1: the team is one workgroup).  kW > 0 fixes the
 // ring width at compile time (the LDS tier's default width); 0 reads it from
-// the layout.
-template <bool L, int NW, int W = 0>
+// the layout.  kDefPen fixes the penalties at WFA2's defaults (x=4, o=6, e=2,
+// match 0); otherwise they come from the launch parameters.
+template <bool L, int NW, int W = 0, bool P = false>
 struct Cfg {
   static constexpr bool kLds = L;
   static constexpr int kNW = NW;
   static constexpr int kW = W;
+  static constexpr bool kDefPen = P;
 };
 constexpr int kLdsDefaultWidth = 32;   // LDS-tier ring width unless tuned otherwise
+constexpr int kDefX = 4, kDefO = 6, kDefE = 2;   // WFA2's default gap-affine penalties
+constexpr int kDefScope = (kDefO + kDefE > kDefX ? kDefO + kDefE : kDefX) + 1;   // ring depth 9
+constexpr int kDefGmask = 1;   // every reachable score is even
 
 // Workspace layout of one aligner (a wave, or a team of waves), in int32 units
 // from its workspace base.  The ring / arena is shared by the team; every wave

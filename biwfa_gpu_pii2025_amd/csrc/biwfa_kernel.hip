@@ -1868,7 +1868,11 @@ void biwfa_persistent(KParams p) {
   c.tred = tred;
   c.tph = 0;
   c.L = p.L;
-  c.x = p.x; c.o = p.o; c.e = p.e; c.scope = p.scope;
+  if constexpr (K::kDefPen) {   // compile-time constants; the launch checks the aligner's penalties
+    c.x = kDefX; c.o = kDefO; c.e = kDefE; c.scope = kDefScope;
+  } else {
+    c.x = p.x; c.o = p.o; c.e = p.e; c.scope = p.scope;
+  }
   c.max_steps = p.max_steps;
   if (K::kLds) {
     c.upen = lane == 0 ? p.um : lane == 1 ? p.ux : lane == 2 ? p.uo : p.ue;
@@ -1877,7 +1881,7 @@ void biwfa_persistent(KParams p) {
     const int cq = lane <= 1 ? CM : lane == 2 ? CI : CD;
     int m0 = -(int)(cq == CM), m1 = -(int)(cq == CI), m2 = -(int)(cq == CD);
     int al = lane == 0 ? 0 : lane == 2 ? 1 : -1, ah = lane == 0 ? 0 : lane == 3 ? -1 : 1;
-    int qd = lane == 0 ? p.x : lane == 1 ? p.o + p.e : p.e, qc = cq;
+    int qd = lane == 0 ? c.x : lane == 1 ? c.o + c.e : c.e, qc = cq;
     const int oc = lane % 3;
     int o0 = -(int)(oc == 0), o1 = -(int)(oc == 1), o2 = -(int)(oc == 2);
     asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(al), "+v"(ah));
@@ -1889,7 +1893,7 @@ void biwfa_persistent(KParams p) {
   } else {
     c.um = p.um; c.ux = p.ux; c.uo = p.uo; c.ue = p.ue;
   }
-  c.gmask = p.gmask;
+  c.gmask = K::kDefPen ? kDefGmask : p.gmask;
   c.epoch = (uni(*(volatile const int32_t*)p.calls) << 3) | p.epoch;
   const int count = uni(*(volatile const int32_t*)p.count);
   if constexpr (!K::kLds) {
@@ -1989,6 +1993,7 @@ __global__ __launch_bounds__(256) void biwfa_compact(const uint32_t* runs, const
 
 template __global__ void biwfa_persistent<Cfg<true, 1>>(KParams);
 template __global__ void biwfa_persistent<Cfg<true, 1, kLdsDefaultWidth>>(KParams);
+template __global__ void biwfa_persistent<Cfg<true, 1, kLdsDefaultWidth, true>>(KParams);
 template __global__ void biwfa_persistent<Cfg<false, 1>>(KParams);
 template __global__ void biwfa_persistent<Cfg<false, 2>>(KParams);
 template __global__ void biwfa_persistent<Cfg<false, 4>>(KParams);

@@ -331,6 +331,24 @@ static int null_score_mask(const Penalties& q) {
   return g > 1 ? g - 1 : 0;
 }
 
+// WFA2's default penalties (x=4, o=6, e=2), which the LDS kernel has a
+// compile-time variant for.
+static bool default_penalties(const Penalties& q) {
+  return q.x == bw::kDefX && q.o == bw::kDefO && q.e == bw::kDefE && q.scope == bw::kDefScope &&
+         null_score_mask(q) == bw::kDefGmask;
+}
+
+// LDS-tier kernel instantiation: ring width 32 and the default penalties as
+// compile-time constants where they apply (fewer scalar registers live across
+// the recursion), else the runtime-parameter kernel.
+static const void* lds_kernel_for(int wcap, bool defpen) {
+  if (wcap == bw::kLdsDefaultWidth && defpen)
+    return reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<true, 1, bw::kLdsDefaultWidth, true>>);
+  if (wcap == bw::kLdsDefaultWidth)
+    return reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<true, 1, bw::kLdsDefaultWidth>>);
+  return reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<true, 1>>);
+}
+
 // Upper bound of a base-case score: 250 below a breakpoint, 500 on the
 // end-reached fallback, and the max(p,t) <= 100 whole-pair case
 // (min(p,t)*x + o + |p-t|*e, the bound WFA2 sizes its wavefront table with,
@@ -367,9 +385,7 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
     t.waves_per_block = bw::kWavesPerBlock;
     t.dyn_lds = (size_t)t.L.total * 4 * t.waves_per_block;
     int nb = 0;
-    const void* lds_kernel = lds_wcap == bw::kLdsDefaultWidth
-                                 ? reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<true, 1, bw::kLdsDefaultWidth>>)
-                                 : reinterpret_cast<const void*>(&bw::biwfa_persistent<bw::Cfg<true, 1>>);
+    const void* lds_kernel = lds_kernel_for(lds_wcap, default_penalties(q));
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lds_kernel, bw::kWave * t.waves_per_block, t.dyn_lds) !=
             hipSuccess ||
         nb <= 0)
@@ -657,11 +673,9 @@ static int launch_sequence(bw_aligner_t* a, bw_dbatch_t* b, bool events) {
     p.gws_stride = T.stride;
     p.seq_lds_words = T.seq_lds_words;
     if (T.lds) {
-      if (T.L.wcap == bw::kLdsDefaultWidth)   // the ring width as a compile-time constant
-        bw::biwfa_persistent<bw::Cfg<true, 1, bw::kLdsDefaultWidth>>
-            <<<dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), T.dyn_lds, s>>>(p);
-      else
-        bw::biwfa_persistent<bw::Cfg<true, 1>><<<dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), T.dyn_lds, s>>>(p);
+      void* args[] = {&p};
+      HIP_TRY(hipLaunchKernel(lds_kernel_for(T.L.wcap, default_penalties(a->pen)), dim3(T.blocks),
+                              dim3(bw::kWave * T.waves_per_block), args, T.dyn_lds, s));
       HIP_TRY(hipGetLastError());
     } else {
       void* args[] = {&p};
