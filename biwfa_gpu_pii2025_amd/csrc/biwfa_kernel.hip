@@ -286,6 +286,12 @@ DI int ws_data(const Ctx<K>& c) { return K::kLds ? 0 : c.L.data; }
 // ring width (a power of two): compile-time in the default LDS configuration
 template <class K>
 DI int wcap(const Ctx<K>& c) { return K::kW > 0 ? K::kW : c.L.wcap; }
+// base-case arena capacity: the LDS tier's arena is its two rings' space
+// (2 x scope x 3 x wcap), compile-time when both the width and the penalties are
+template <class K>
+DI int arena_cap(const Ctx<K>& c) {
+  return (K::kLds && K::kW > 0 && K::kDefPen) ? 2 * kDefScope * 3 * K::kW : c.L.arena_cap;
+}
 
 // Offsets in a wave's private region of the backtrace run buffer and the
 // recursion stack (compile-time in the LDS kernel: its region holds no
@@ -606,7 +612,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     mask = wcap(c) - 1;
   } else {
     const int need = width * (1 + (int)has_i + (int)has_d);
-    if (c.arena_top + need > c.L.arena_cap) { c.spill = 1; return 0; }
+    if (c.arena_top + need > arena_cap(c)) { c.spill = 1; return 0; }
     int top = ws_data(c) + c.arena_top;
     bm = top - lo; top += width;
     bi = top - lo; if (has_i) top += width;

@@ -502,8 +502,10 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
   {
     // the LDS kernel takes these layout values as compile-time constants
     const bw::WsLayout& L0 = b->tiers[0].L;
+    const bool fixed = L0.wcap == bw::kLdsDefaultWidth && default_penalties(a->pen);   // compiled-in variant
     if (L0.data != 0 || L0.hcap != bw::kLdsHcap || L0.rcap != bw::kLdsRcap || L0.scap != bw::kLdsScap ||
-        L0.runs_tmp != 0 || L0.stack != bw::kLdsRcap) {
+        L0.runs_tmp != 0 || L0.stack != bw::kLdsRcap ||
+        (fixed && L0.arena_cap != 2 * bw::kDefScope * 3 * bw::kLdsDefaultWidth)) {
       set_error("LDS-tier layout does not match the kernel's constants");
       bw_dbatch_free(b);
       return BW_STATUS_INTERNAL;
