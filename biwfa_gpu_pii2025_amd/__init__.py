@@ -27,6 +27,11 @@ STATUS_MAX_STEPS_REACHED = -100
 STATUS_OOM = -200
 STATUS_UNATTAINABLE = -300
 STATUS_DEVICE_CAPACITY = -400
+STATUS_HIP_ERROR = -401
+STATUS_INVALID_ARG = -402
+STATUS_NO_DEVICE = -403
+STATUS_BAD_SEQUENCE = -404
+STATUS_INTERNAL = -405
 
 SCOPE_ALIGNMENT = 0
 SCOPE_SCORE = 1
@@ -129,6 +134,7 @@ def lib():
                                   ctypes.POINTER(_Tuning)]
         L.bw_set_progress.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.bw_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.bw_test_set_ready_guard.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
         L.bw_dataset_lengths.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_generate.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]
@@ -442,6 +448,15 @@ class WavefrontAligner:
         if st != 0:
             raise BiwfaError("bw_set_graph failed (%d)" % st)
         self._graph = bool(enable)
+
+    # -- failure-path testing (bw_test_set_ready_guard) --
+    def set_ready_guard(self, ticks=0, withhold=False):
+        """Bounds the wait for a claimed shared task's ready flag (ticks of the 100 MHz
+        device clock; 0 = the 60 s default) and optionally withholds the flags, so
+        that every such wait gives up (the tier then ends with BW_STATUS_INTERNAL)."""
+        st = self._L.bw_test_set_ready_guard(self._h, int(ticks), 1 if withhold else 0)
+        if st != 0:
+            raise BiwfaError("bw_test_set_ready_guard failed (%d)" % st)
 
     # -- launch tuning (speed only; results never change) --
     @property

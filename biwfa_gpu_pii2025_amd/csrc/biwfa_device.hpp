@@ -121,6 +121,10 @@ struct KParams {
   int32_t* pend;                 // per pair: unfinished sub-problems
   unsigned long long* pfail;     // per pair: leftmost failure, (pb+tb) << 16 | -status
   int32_t* pflag;                // per pair: 1 workspace capacity exceeded, 2 run overflow
+  int32_t task_min_length;       // pairs with p + t below this run depth first on one
+                                 // aligner (no shared tasks); INT32_MAX: never shared
+  int64_t ready_ticks;           // longest wait (100 MHz ticks) for a claimed item's ready flag
+  int32_t withhold_ready;        // failure-path test: publish items without their ready flag
   WsLayout L;
 };
 

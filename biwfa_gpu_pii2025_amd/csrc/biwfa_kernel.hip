@@ -1522,13 +1522,13 @@ DI int align_pair(Ctx<K>& c, int pair, int plen, int tlen, int score_only, int* 
 
 constexpr int kFlagSpill = 1, kFlagRuns = 2;
 // A claimed item's producer writes it within microseconds of claiming its slot;
-// a taker that waits longer than this for the ready flag gives up and raises
-// the tier's error flag (a defect, never seen in a correct run).  100 MHz clock.
-constexpr uint64_t kReadyTicks = 60ull * 100000000ull;
-// Pairs shorter than this (p + t) are aligned by one aligner from start to end
-// with the runs emitted in order: sharing their halves costs more (zeroing,
-// merging, agent-scope fences) than it saves.
-constexpr int kTaskMinLength = 4096;
+// a taker that waits longer than KParams::ready_ticks (100 MHz clock, 60 s by
+// default) for the ready flag gives up and raises the tier's error flag (a
+// defect, never seen in a correct run).  Every aligner of the tier then leaves
+// its work loop at its next poll, so the launch ends and the host reports the
+// flag.  Pairs shorter than KParams::task_min_length (p + t) are aligned by one
+// aligner from start to end with the runs emitted in order: sharing their
+// halves costs more (zeroing, merging, agent-scope fences) than it saves.
 
 DI int atomic_ld(const int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
@@ -1584,7 +1584,7 @@ DI int publish(Ctx<K>& c, const KParams& p, const Sub& s) {
       it[1] = s.pair; it[2] = s.pb; it[3] = s.pe; it[4] = s.tb; it[5] = s.te;
       it[6] = s.cb; it[7] = s.ce; it[8] = s.rem; it[9] = s.lvl;
       __threadfence();
-      __hip_atomic_store(it, c.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (!p.withhold_ready) __hip_atomic_store(it, c.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       ok = 1;
     }
   }
@@ -1739,7 +1739,7 @@ DI void start_pair(Ctx<K>& c, const KParams& p, int pair, int& staged, int32_t* 
     }
     return;
   }
-  if (plen + tlen < kTaskMinLength) {   // short pair: depth first on this aligner
+  if (plen + tlen < p.task_min_length) {   // short pair: depth first on this aligner
     if (pair != staged) {
       stage_pair(c, p, pair, lds, wv);
       staged = pair;
@@ -1798,6 +1798,9 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
     int kind = 0, arg = 0;   // 1: queued item (arg = slot), 2: new pair (arg = id), -1: done
     if (c.leader) {
       for (;;) {
+        // an aligner gave up on a claimed item: its pair can never finish, so
+        // the in-flight count never drops; everyone leaves and the host reports it
+        if (atomic_ld(&q[5]) != 0) { kind = -1; break; }
         const int h = atomic_ld(&q[2]);
         const int t = imin(atomic_ld(&q[3]), p.item_cap);
         if (h < t) {
@@ -1832,7 +1835,7 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
       if (c.leader) {
         const uint64_t t0 = wall_clock64();
         while (atomic_ld(it) != c.epoch) {
-          if (wall_clock64() - t0 > kReadyTicks) { ready = 0; break; }
+          if (wall_clock64() - t0 > (uint64_t)p.ready_ticks || atomic_ld(&q[5]) != 0) { ready = 0; break; }
           __builtin_amdgcn_s_sleep(2);
         }
       }

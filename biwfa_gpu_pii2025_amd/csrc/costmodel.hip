@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "biwfa_mi355x.h"
@@ -455,6 +456,22 @@ int bw_batch_features(int device, const bw_batch_t* b, float* X) {
   if (!b || b->num_pairs < 0 || (b->num_pairs > 0 && !X)) return BW_STATUS_INVALID_ARG;
   const int64_t n = b->num_pairs;
   if (n == 0) return 0;
+  if (n > INT32_MAX || !b->words || !b->pattern_word_offset || !b->text_word_offset || !b->pattern_length ||
+      !b->text_length) {
+    g_err = "bw_batch_features: invalid batch";
+    return BW_STATUS_INVALID_ARG;
+  }
+  // the same layout rules as bw_dbatch_upload: one readable word before every
+  // sequence and two after it, so the kernel's windows stay inside `words`
+  for (int64_t i = 0; i < n; ++i) {
+    const int p = b->pattern_length[i], t = b->text_length[i];
+    const int64_t pw = b->pattern_word_offset[i], tw = b->text_word_offset[i];
+    if (p < 0 || t < 0 || pw < 1 || tw < 1 || pw + (p + 31) / 32 + 2 > b->num_words ||
+        tw + (t + 31) / 32 + 2 > b->num_words) {
+      g_err = "bw_batch_features: pair " + std::to_string(i) + " has a bad length or word offset";
+      return BW_STATUS_INVALID_ARG;
+    }
+  }
   CM_TRY(hipSetDevice(device));
   DevBuf<uint64_t> dW;
   DevBuf<int64_t> dPo, dTo;

@@ -186,14 +186,23 @@ struct bw_dbatch {
   int32_t* d_pflag = nullptr;
   int64_t gws_bytes = 0;
   std::vector<int32_t> tier_count;   // initial pairs per tier (last schedule)
-  // graph mode: the launch sequence captured for one aligner, replayed on later calls
+  // The aligner that uploaded the batch: its tiers were planned for that
+  // aligner's penalties, tuning and device, so only it may align, schedule or
+  // download the batch (bw_dbatch_align / bw_set_schedule / bw_dbatch_download).
+  uint64_t owner = 0;
+  // graph mode: the owner's launch sequence, captured once and replayed
   hipGraphExec_t gexec = nullptr;
-  const void* g_aligner = nullptr;
 };
 
 constexpr int kMaxTiers = 8;
+// Longest wait for a claimed shared task's ready flag: 60 s of the 100 MHz clock.
+constexpr int64_t kReadyTicksDefault = 60ll * 100000000ll;
+// Pairs with p + t below this run depth first on one aligner of a global tier.
+constexpr int kTaskMinLength = 4096;
+std::atomic<uint64_t> g_next_aligner_id{1};
 
 struct bw_aligner {
+  uint64_t id = 0;                 // unique over the process (device-batch ownership)
   bw_attr_t attr{};
   Penalties pen{};
   int dev = 0, cu_count = 0;
@@ -209,6 +218,9 @@ struct bw_aligner {
   void* progress_user = nullptr;
   int32_t* d_counters = nullptr;   // per tier: {queue, count, head, tail, in-flight task pairs, error},
                                    // then the call counter the ready-flag epochs derive from
+  const int32_t* last_counters = nullptr;   // counters of the last launch sequence (d_counters or scratch)
+  int64_t ready_ticks = kReadyTicksDefault; // bound of a shared-task ready-flag wait (100 MHz ticks)
+  int withhold_ready = 0;                   // failure-path test hook (bw_test_set_ready_guard)
   int graph = 0;                   // replay a captured launch sequence (bw_set_graph)
   bool last_graph = false;         // the last sequence was a graph replay (no per-tier events)
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
@@ -280,6 +292,7 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
     return nullptr;
   }
   bw_aligner* a = new bw_aligner();
+  a->id = g_next_aligner_id.fetch_add(1);
   a->attr = attr;
   a->pen = pen;
   a->dev = attr.device;
@@ -367,34 +380,102 @@ static int team_waves(const bw_aligner_t* a, int wcap) {
   return 4;
 }
 
-static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b) {
-  std::vector<Tier> tiers;
+// Tier 0: the whole workspace and both sequences in LDS, one wave per pair,
+// ring width 32 and sequences up to 512 bases staged unless tuned otherwise.
+static Tier make_lds_tier(const bw_aligner_t* a) {
+  const Penalties& q = a->pen;
+  const bw_tuning_t& tu = a->tuning;
+  Tier t;
+  t.lds = true;
+  t.seq_cap = tu.lds_seq_cap > 0 ? tu.lds_seq_cap : 512;
+  const int seq_words = (t.seq_cap + 31) / 32 + 3;
+  const int lds_wcap = tu.lds_ring_width > 0 ? tu.lds_ring_width : bw::kLdsDefaultWidth;
+  t.L = make_layout(q.scope, lds_wcap, 0, bw::kLdsHcap, bw::kLdsRcap, bw::kLdsScap, seq_words, false, 1);
+  t.waves_per_block = bw::kWavesPerBlock;
+  t.dyn_lds = (size_t)t.L.total * 4 * t.waves_per_block;
+  int nb = 0;
+  const void* lds_kernel = lds_kernel_for(lds_wcap, default_penalties(q));
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lds_kernel, bw::kWave * t.waves_per_block, t.dyn_lds) !=
+          hipSuccess ||
+      nb <= 0)
+    nb = std::max<int>(1, (int)((160 * 1024) / t.dyn_lds));
+  if (a->attr.waves_per_cu > 0) nb = std::max(1, std::min(nb, a->attr.waves_per_cu / t.waves_per_block));
+  t.blocks = a->cu_count * nb;
+  t.chunk = tu.queue_chunk[0] > 0 ? tu.queue_chunk[0] : a->attr.queue_chunk > 0 ? a->attr.queue_chunk : 4;
+  return t;
+}
+
+// Global tier gi (ring width wcap; `last`: the widest tier, whose base-case
+// arena holds the worst case of the batch) for pairs of p + t <= max_pt and
+// sequences of at most max_seq_words words (with pads).  `budget` bounds the
+// bytes of all its workspace slabs together.  nw > 0 fixes the team size.
+static Tier make_global_tier(const bw_aligner_t* a, int gi, int wcap, bool last, int max_pt, int max_seq_words,
+                             int64_t budget, int nw = 0) {
   const Penalties& q = a->pen;
   const int scope = q.scope;
   const int smax = base_score_bound(q);
   const int g = null_score_mask(q) + 1;
   const bw_tuning_t& tu = a->tuning;
-  // tier 0: LDS workspace, ring width 32, sequences up to 512 bases staged
-  {
-    Tier t;
-    t.lds = true;
-    t.seq_cap = tu.lds_seq_cap > 0 ? tu.lds_seq_cap : 512;
-    const int seq_words = (t.seq_cap + 31) / 32 + 3;
-    const int lds_wcap = tu.lds_ring_width > 0 ? tu.lds_ring_width : bw::kLdsDefaultWidth;
-    t.L = make_layout(scope, lds_wcap, 0, bw::kLdsHcap, bw::kLdsRcap, bw::kLdsScap, seq_words, false, 1);
+  Tier t;
+  t.lds = false;
+  t.nw = nw > 0 ? nw : tu.team_waves[gi] > 0 ? tu.team_waves[gi] : team_waves(a, wcap);
+  const int width = std::min(2 * smax / q.e + 3, max_pt + 3);
+  const int64_t worst = (int64_t)3 * ((smax + g) / g) * width;
+  const int64_t arena = last ? worst : std::min<int64_t>(worst, (int64_t)3 * ((smax + g) / g) * std::min(wcap, width));
+  const int steps = std::min(max_pt, smax / std::max(1, std::min(q.x, q.e)) + 1);
+  t.L = make_layout(scope, wcap, (int)std::min<int64_t>(arena, (int64_t)1 << 30), smax + 2,
+                    2 * steps + 16, 64, 0, true, t.nw);
+  t.stride = t.L.total;
+  const int64_t per_aligner = t.stride * 4;
+  const void* kern = team_kernel(t.nw);
+  // Pairs are staged in LDS when the batch's longest sequence fits the
+  // aligner's share of the CU's LDS (the kernels run 4 waves per SIMD, i.e.
+  // 16 waves per CU).
+  const int64_t lds_cu = 160 * 1024;
+  if (t.nw == 1) {
     t.waves_per_block = bw::kWavesPerBlock;
-    t.dyn_lds = (size_t)t.L.total * 4 * t.waves_per_block;
+    t.seq_lds_words = (max_seq_words * 16 <= lds_cu / 16) ? max_seq_words : 0;
+    t.dyn_lds = (size_t)t.seq_lds_words * 16 * bw::kWavesPerBlock;
+    const int64_t waves = std::max<int64_t>(bw::kWavesPerBlock,
+                                            std::min<int64_t>(budget / per_aligner, (int64_t)a->cu_count * 16));
+    t.blocks = (int)std::max<int64_t>(1, waves / bw::kWavesPerBlock);
+    t.aligners = (int64_t)t.blocks * bw::kWavesPerBlock;
+  } else {
+    // one team per workgroup
+    t.waves_per_block = t.nw;
+    const int teams_cu = std::max(1, 16 / t.nw);
+    const int64_t red_bytes = (int64_t)2 * t.nw * bw::kRedSlots * 4;
+    const int64_t share = lds_cu / teams_cu - red_bytes;
+    t.seq_lds_words = (max_seq_words * 16 <= share) ? max_seq_words : 0;
+    t.dyn_lds = (size_t)t.seq_lds_words * 16;
     int nb = 0;
-    const void* lds_kernel = lds_kernel_for(lds_wcap, default_penalties(q));
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lds_kernel, bw::kWave * t.waves_per_block, t.dyn_lds) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, bw::kWave * t.nw, t.dyn_lds) != hipSuccess ||
         nb <= 0)
-      nb = std::max<int>(1, (int)((160 * 1024) / t.dyn_lds));
-    if (a->attr.waves_per_cu > 0) nb = std::max(1, std::min(nb, a->attr.waves_per_cu / t.waves_per_block));
-    t.blocks = a->cu_count * nb;
-    t.chunk = tu.queue_chunk[0] > 0 ? tu.queue_chunk[0] : a->attr.queue_chunk > 0 ? a->attr.queue_chunk : 4;
-    tiers.push_back(t);
+      nb = 1;
+    const int64_t teams = std::max<int64_t>(1, std::min<int64_t>(budget / per_aligner, (int64_t)a->cu_count * nb));
+    t.blocks = (int)teams;
+    t.aligners = teams;
   }
+  t.chunk = tu.queue_chunk[gi + 1] > 0 ? tu.queue_chunk[gi + 1] : 1;
+  return t;
+}
+
+// The LDS kernel takes these layout values as compile-time constants.
+static int check_lds_layout(const bw_aligner_t* a, const Tier& t0) {
+  const bw::WsLayout& L0 = t0.L;
+  const bool fixed = L0.wcap == bw::kLdsDefaultWidth && default_penalties(a->pen);   // compiled-in variant
+  if (L0.data != 0 || L0.hcap != bw::kLdsHcap || L0.rcap != bw::kLdsRcap || L0.scap != bw::kLdsScap ||
+      L0.runs_tmp != 0 || L0.stack != bw::kLdsRcap ||
+      (fixed && L0.arena_cap != 2 * bw::kDefScope * 3 * bw::kLdsDefaultWidth)) {
+    set_error("LDS-tier layout does not match the kernel's constants");
+    return BW_STATUS_INTERNAL;
+  }
+  return 0;
+}
+
+static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b) {
+  std::vector<Tier> tiers;
+  tiers.push_back(make_lds_tier(a));
   // global tiers: ring width 256, 1024, ... up to the batch maximum
   const int wmax = next_pow2(std::max(b->max_pt + 8, 256));
   const int64_t budget = (int64_t)24 << 30;
@@ -402,49 +483,7 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
   for (int w = 256;; w *= 4) {
     const bool last = w >= wmax || (int)tiers.size() + 1 >= kMaxTiers;
     const int wcap = last ? wmax : w;
-    Tier t;
-    t.lds = false;
-    const int gi = (int)tiers.size() - 1;
-    t.nw = tu.team_waves[gi] > 0 ? tu.team_waves[gi] : team_waves(a, wcap);
-    const int width = std::min(2 * smax / q.e + 3, b->max_pt + 3);
-    const int64_t worst = (int64_t)3 * ((smax + g) / g) * width;
-    const int64_t arena = last ? worst : std::min<int64_t>(worst, (int64_t)3 * ((smax + g) / g) * std::min(wcap, width));
-    const int steps = std::min(b->max_pt, smax / std::max(1, std::min(q.x, q.e)) + 1);
-    t.L = make_layout(scope, wcap, (int)std::min<int64_t>(arena, (int64_t)1 << 30), smax + 2,
-                      2 * steps + 16, 64, 0, true, t.nw);
-    t.stride = t.L.total;
-    const int64_t per_aligner = t.stride * 4;
-    const void* kern = team_kernel(t.nw);
-    // Pairs are staged in LDS when the batch's longest sequence fits the
-    // aligner's share of the CU's LDS (the kernels run 4 waves per SIMD, i.e.
-    // 16 waves per CU).
-    const int64_t lds_cu = 160 * 1024;
-    if (t.nw == 1) {
-      t.waves_per_block = bw::kWavesPerBlock;
-      t.seq_lds_words = (max_seq_words * 16 <= lds_cu / 16) ? max_seq_words : 0;
-      t.dyn_lds = (size_t)t.seq_lds_words * 16 * bw::kWavesPerBlock;
-      const int64_t waves = std::max<int64_t>(bw::kWavesPerBlock,
-                                              std::min<int64_t>(budget / per_aligner, (int64_t)a->cu_count * 16));
-      t.blocks = (int)std::max<int64_t>(1, waves / bw::kWavesPerBlock);
-      t.aligners = (int64_t)t.blocks * bw::kWavesPerBlock;
-    } else {
-      // one team per workgroup
-      t.waves_per_block = t.nw;
-      const int teams_cu = std::max(1, 16 / t.nw);
-      const int64_t red_bytes = (int64_t)2 * t.nw * bw::kRedSlots * 4;
-      const int64_t share = lds_cu / teams_cu - red_bytes;
-      t.seq_lds_words = (max_seq_words * 16 <= share) ? max_seq_words : 0;
-      t.dyn_lds = (size_t)t.seq_lds_words * 16;
-      int nb = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, bw::kWave * t.nw, t.dyn_lds) != hipSuccess ||
-          nb <= 0)
-        nb = 1;
-      const int64_t teams = std::max<int64_t>(1, std::min<int64_t>(budget / per_aligner, (int64_t)a->cu_count * nb));
-      t.blocks = (int)teams;
-      t.aligners = teams;
-    }
-    t.chunk = tu.queue_chunk[gi + 1] > 0 ? tu.queue_chunk[gi + 1] : 1;
-    tiers.push_back(t);
+    tiers.push_back(make_global_tier(a, (int)tiers.size() - 1, wcap, last, b->max_pt, max_seq_words, budget));
     if (last || (int)tiers.size() >= kMaxTiers) break;
   }
   return tiers;
@@ -498,18 +537,11 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
     runs_off[i + 1] = runs_off[i] + (int64_t)p + t + 1;
   }
   b->runs_cap = runs_off[n];
+  b->owner = a->id;
   b->tiers = plan_tiers(a, b);
-  {
-    // the LDS kernel takes these layout values as compile-time constants
-    const bw::WsLayout& L0 = b->tiers[0].L;
-    const bool fixed = L0.wcap == bw::kLdsDefaultWidth && default_penalties(a->pen);   // compiled-in variant
-    if (L0.data != 0 || L0.hcap != bw::kLdsHcap || L0.rcap != bw::kLdsRcap || L0.scap != bw::kLdsScap ||
-        L0.runs_tmp != 0 || L0.stack != bw::kLdsRcap ||
-        (fixed && L0.arena_cap != 2 * bw::kDefScope * 3 * bw::kLdsDefaultWidth)) {
-      set_error("LDS-tier layout does not match the kernel's constants");
-      bw_dbatch_free(b);
-      return BW_STATUS_INTERNAL;
-    }
+  if (const int st = check_lds_layout(a, b->tiers[0])) {
+    bw_dbatch_free(b);
+    return st;
   }
   auto fail = [&](hipError_t e) {
     set_error(std::string("device allocation/copy failed: ") + hipGetErrorString(e));
@@ -569,8 +601,19 @@ bw_dbatch_t* bw_dbatch_upload(bw_aligner_t* a, const bw_batch_t* in) {
   return b;
 }
 
+// The batch's tiers were planned for the aligner that uploaded it.
+static int check_owner(const bw_aligner_t* a, const bw_dbatch_t* b) {
+  if (b->owner != a->id) {
+    set_error("the device batch was uploaded by another aligner (its tiers are planned for that "
+              "aligner's penalties, tuning and device)");
+    return BW_STATUS_INVALID_ARG;
+  }
+  return 0;
+}
+
 int bw_set_schedule(bw_aligner_t* a, bw_dbatch_t* b, int schedule, const float* predicted_score) {
   if (!a || !b) return BW_STATUS_INVALID_ARG;
+  if (const int st = check_owner(a, b)) return st;
   if (schedule == BW_SCHEDULE_LEARNED && !predicted_score) {
     set_error("the learned schedule needs one predicted score per pair");
     return BW_STATUS_INVALID_ARG;
@@ -636,6 +679,9 @@ static bw::KParams base_params(const bw_aligner_t* a, const bw_dbatch_t* b) {
   p.max_steps = a->attr.max_alignment_steps > 0 ? a->attr.max_alignment_steps : INT32_MAX;
   p.score = b->d_score; p.wf_score = b->d_wf; p.status = b->d_status;
   p.runs = b->d_runs; p.runs_off = b->d_runs_off; p.nruns = b->d_nruns;
+  p.task_min_length = kTaskMinLength;
+  p.ready_ticks = a->ready_ticks;
+  p.withhold_ready = a->withhold_ready;
   return p;
 }
 
@@ -694,16 +740,14 @@ static int launch_sequence(bw_aligner_t* a, bw_dbatch_t* b, bool events) {
 
 int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
   if (!a || !b) return BW_STATUS_INVALID_ARG;
+  if (const int st = check_owner(a, b)) return st;
   HIP_TRY(hipSetDevice(a->dev));
   if (b->n == 0) return 0;
   hipStream_t s = a->stream;
   const int nt = (int)b->tiers.size();
+  a->last_counters = a->d_counters;
   if (a->graph) {
-    if (!b->gexec || b->g_aligner != a) {
-      if (b->gexec) {
-        HIP_TRY(hipGraphExecDestroy(b->gexec));
-        b->gexec = nullptr;
-      }
+    if (!b->gexec) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       const int st = launch_sequence(a, b, false);
       hipGraph_t g = nullptr;
@@ -716,7 +760,6 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
       const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       HIP_TRY(ei);
-      b->g_aligner = a;
     }
     HIP_TRY(hipEventRecord(a->ev[0], s));
     HIP_TRY(hipEventRecord(a->ev[1], s));
@@ -754,8 +797,10 @@ int bw_synchronize(bw_aligner_t* a) {
     float ms = 0.f;
     a->tier_ms[t] = !a->last_graph && hipEventElapsedTime(&ms, a->tev[t], a->tev[t + 1]) == hipSuccess ? ms : 0.f;
   }
-  int32_t cnt[bw::kQctlInts * kMaxTiers];
-  HIP_TRY(hipMemcpy(cnt, a->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  int32_t cnt[bw::kQctlInts * kMaxTiers] = {};
+  if (a->last_counters)
+    HIP_TRY(hipMemcpy(cnt, a->last_counters, sizeof(int32_t) * bw::kQctlInts * a->last_ntiers,
+                      hipMemcpyDeviceToHost));
   for (int t = 0; t < kMaxTiers; ++t) a->tier_pairs[t] = t < a->last_ntiers ? cnt[bw::kQctlInts * t + 1] : 0;
   for (int t = 0; t < a->last_ntiers; ++t) {
     if (cnt[bw::kQctlInts * t + 5] != 0) {
@@ -777,6 +822,7 @@ int bw_last_timing(const bw_aligner_t* a, float* total_ms, float* main_ms, float
 
 int bw_dbatch_download(bw_aligner_t* a, bw_dbatch_t* b, bw_result_t* r) {
   if (!a || !b || !r) return BW_STATUS_INVALID_ARG;
+  if (const int st = check_owner(a, b)) return st;
   HIP_TRY(hipSetDevice(a->dev));
   const int64_t n = b->n;
   hipStream_t s = a->stream;
@@ -1140,6 +1186,16 @@ int bw_align_pair(const char* pattern, int plen, const char* text, int tlen, int
     std::memcpy(ops, tl_aligner->ops.data(), tl_aligner->ops.size());
   }
   return st;
+}
+
+
+// ---------------- failure-path test hook ----------------
+
+int bw_test_set_ready_guard(bw_aligner_t* a, int64_t ready_ticks, int withhold_ready) {
+  if (!a || ready_ticks < 0) return BW_STATUS_INVALID_ARG;
+  a->ready_ticks = ready_ticks > 0 ? ready_ticks : kReadyTicksDefault;
+  a->withhold_ready = withhold_ready ? 1 : 0;
+  return 0;
 }
 
 }  // extern "C"

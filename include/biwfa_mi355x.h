@@ -257,6 +257,18 @@ int bw_set_progress(bw_aligner_t* aligner, bw_progress_fn fn, void* user);
  * (src/biWFA.cu:935-1004). */
 int bw_set_graph(bw_aligner_t* aligner, int enable);
 
+/* ---------------- failure-path testing ---------------- */
+
+/* No counterpart in the reference.  For tests of the termination guard only:
+ * `ready_ticks` (> 0; 0 restores the default of 60 s) bounds how long an
+ * aligner of a global tier waits, in ticks of the 100 MHz device clock, for the
+ * ready flag of a shared recursion task it has claimed; `withhold_ready` != 0
+ * publishes such tasks without their ready flag, so every wait for one gives
+ * up.  A wait that gives up raises the tier's error flag, every aligner of the
+ * tier then leaves its work loop, and bw_synchronize returns
+ * BW_STATUS_INTERNAL.  Affects later bw_dbatch_align / bw_align_batch calls. */
+int bw_test_set_ready_guard(bw_aligner_t* aligner, int64_t ready_ticks, int withhold_ready);
+
 /* ---------------- (v) learned cost model ---------------- */
 
 /* No counterpart in the reference: a 4 -> 64 -> 64 -> 1 ReLU MLP (BF16 weights,

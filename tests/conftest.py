@@ -26,6 +26,16 @@ PENALTY_SETS = {
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+    config.addinivalue_line("markers", "slow: CPU checks of several minutes; run with BIWFA_RUN_SLOW=1")
+
+
+def pytest_collection_modifyitems(config, items):
+    if os.environ.get("BIWFA_RUN_SLOW") == "1":
+        return
+    skip = pytest.mark.skip(reason="slow CPU check (several minutes): set BIWFA_RUN_SLOW=1")
+    for item in items:
+        if "slow" in item.keywords:
+            item.add_marker(skip)
 
 
 def _ensure_oracle():

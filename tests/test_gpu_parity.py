@@ -78,6 +78,27 @@ def test_configs_vs_oracle(aligners, oracle, name, n):
     assert np.array_equal(res.wf_score, ref.wf_score)
 
 
+def test_c5_real_shape_vs_wfa2_golden(aligners):
+    """The first 4 pairs of the divergent-long workload at their real shape (100 kbp,
+    15,000 edit events with 10% long indel runs, s ~ 1.9e5, ~11 recursion levels,
+    every pair in the widest tier) against WFA2-lib's own output
+    (tests/golden/c5, written by tests/golden/make_c5_golden.py): score, CIGAR and
+    the level-0 breakpoint score."""
+    import gzip
+    import json
+    import os
+    d = os.path.join(os.path.dirname(__file__), "golden", "c5")
+    with gzip.open(os.path.join(d, "c5_first4.alg.gz"), "rt") as fh:
+        gold = [line.rstrip("\n").split("\t") for line in fh if line.strip()]
+    meta = json.load(open(os.path.join(d, "c5_first4.json")))["pairs"]
+    ds = bw.generate_dataset(bw.CONFIGS["C5"], num_pairs=len(gold))
+    assert [int(x) for x in ds.t_len] == [m["text_length"] for m in meta]
+    _say("C5 real shape: %d pairs of 100 kbp" % len(gold))
+    res = aligners((0, 4, 6, 2)).align_batch(ds.pack())
+    _compare(res, [int(g[0]) for g in gold], [g[1] for g in gold], "C5-100k")
+    assert [int(v) for v in res.wf_score] == [-m["wfa2_cigar_score"] for m in meta]
+
+
 def test_single_pair_api(aligners, utest_pairs):
     """bw_align / accessors on individual pairs (wavefront_align surface)."""
     gold = golden_alg("")
@@ -218,13 +239,20 @@ def test_max_steps_status(aligners, oracle):
     assert list(res.status) == list(ref.status)
 
 
-def test_tuning_changes_speed_not_results(aligners):
-    """Every team size and LDS ring width gives bit-identical results; the tuning
-    round-trips through the ABI and bad values are rejected."""
+def test_tuning_changes_speed_not_results(aligners, oracle):
+    """Every team size and LDS ring width (the runtime-width LDS kernels 16, 64 and
+    128 included) matches the oracle bit for bit on a mixed batch and on the utest
+    pairs; the tuning round-trips through the ABI and bad values are rejected."""
     _say("tunings")
+    from oracle.oracle import read_seq_file
+    import os
     ds = bw.generate_dataset(bw.CONFIGS["C4"], num_pairs=120)
-    packed = ds.pack()
-    ref = None
+    short = bw.generate_dataset(bw.CONFIGS["C2"], num_pairs=300)
+    utest = read_seq_file(os.path.join(os.path.dirname(__file__), "golden", "wfa2_utest", "wfa.utest.seq"))
+    pairs = ds.pairs() + short.pairs() + utest[:120]
+    packed = bw.pack_pairs(pairs)
+    ref = oracle.align(pairs, threads=16)
+    ref_cigars = [ref.cigar(i) for i in range(len(pairs))]
     for nw, lw in ((1, 16), (2, 32), (4, 64), (8, 32), (16, 128)):
         a = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
         t = {"lds_ring_width": lw, "lds_seq_cap": 0, "team_waves": [nw] * bw.MAX_TIERS,
@@ -232,13 +260,8 @@ def test_tuning_changes_speed_not_results(aligners):
         a.tuning = t
         assert a.tuning["team_waves"] == [nw] * bw.MAX_TIERS and a.tuning["lds_ring_width"] == lw
         res = a.align_batch(packed)
-        assert set(int(x) for x in res.status) == {0}
-        if ref is None:
-            ref = res
-        else:
-            assert np.array_equal(res.score, ref.score)
-            assert np.array_equal(res.cigar_offset, ref.cigar_offset)
-            assert np.array_equal(res.cigar_runs, ref.cigar_runs)
+        _compare(res, ref.alg_score, ref_cigars, "tuning nw=%d lds_ring_width=%d" % (nw, lw))
+        assert np.array_equal(res.wf_score, ref.wf_score)
         a.close()
     a = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
     with pytest.raises(bw.BiwfaError):
@@ -248,13 +271,14 @@ def test_tuning_changes_speed_not_results(aligners):
     a.close()
 
 
-def test_autotune_reports_and_keeps_results(aligners):
+def test_autotune_reports_and_keeps_results(aligners, oracle):
     """bw_autotune times candidates (progress lines), keeps a valid tuning, and the
-    tuned aligner still aligns bit-exactly; per-tier pairs add up to the batch
-    plus the re-runs."""
+    tuned aligner still matches the oracle bit for bit; per-tier pairs add up to
+    the batch plus the re-runs."""
     _say("autotune")
     ds = bw.generate_dataset(bw.CONFIGS["C4"], num_pairs=150)
     packed = ds.pack()
+    oref = oracle.align(ds.pairs(), threads=16)
     a = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
     lines = []
     t = a.autotune(packed, repeats=1, progress=lines.append)
@@ -265,8 +289,7 @@ def test_autotune_reports_and_keeps_results(aligners):
     res = db.download()
     pairs = a.tier_pairs()
     assert sum(pairs) >= len(packed) and pairs[0] + sum(pairs[1:]) == sum(pairs)
-    ref = aligners((0, 4, 6, 2)).align_batch(packed)
-    assert np.array_equal(res.score, ref.score) and np.array_equal(res.cigar_runs, ref.cigar_runs)
+    _compare(res, oref.alg_score, [oref.cigar(i) for i in range(len(packed))], "autotuned")
     db.free()
     a.close()
 
@@ -322,19 +345,21 @@ def test_global_tiers_step_cap_and_score_only(aligners, oracle):
     assert np.array_equal(res.score, ref.alg_score)
 
 
-def test_graph_replay_matches_eager():
+def test_graph_replay_matches_eager(oracle):
     """Graph mode: the captured launch sequence replayed several times, with other
-    eager work and synchronisations in between, gives the eager results every time
-    (scores, CIGARs, statuses), on a batch that spans the LDS tier and the global
-    tiers' shared recursion tasks."""
+    eager work and synchronisations in between, gives the oracle's results every
+    time (scores, CIGARs, statuses), on a batch that spans the LDS tier and the
+    global tiers' shared recursion tasks."""
     _say("graph replay")
     ds = bw.generate_dataset(bw.CONFIGS["C4"], num_pairs=160)
     short = bw.generate_dataset(bw.CONFIGS["C2"], num_pairs=200)
-    packed = bw.pack_pairs(ds.pairs() + short.pairs())
+    pairs = ds.pairs() + short.pairs()
+    packed = bw.pack_pairs(pairs)
     other = bw.generate_dataset(bw.CONFIGS["C1"], num_pairs=300).pack()
     eager = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
     ref = eager.align_batch(packed)
-    assert set(int(s) for s in ref.status) == {0}
+    oref = oracle.align(pairs, threads=16)
+    _compare(ref, oref.alg_score, [oref.cigar(i) for i in range(len(pairs))], "graph-eager")
     ref_other = eager.align_batch(other)
     g = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
     g.graph = True
@@ -360,3 +385,42 @@ def test_graph_replay_matches_eager():
     db.free()
     g.close()
     eager.close()
+
+
+def test_device_batch_belongs_to_its_uploader():
+    """A device batch's tiers are planned for the aligner that uploaded it; another
+    aligner (other penalties) is refused instead of running kernels whose
+    compile-time layout does not match the batch."""
+    _say("owner check")
+    pairs = bw.generate_dataset(bw.CONFIGS["C2"], num_pairs=50).pairs()
+    a = bw.WavefrontAligner(penalties=(0, 2, 3, 1))
+    b = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
+    db = a.upload(bw.pack_pairs(pairs))
+    assert b._L.bw_dbatch_align(b._h, db._h) == bw.STATUS_INVALID_ARG
+    assert b._L.bw_set_schedule(b._h, db._h, bw.SCHEDULE_LENGTH_SORTED, None) == bw.STATUS_INVALID_ARG
+    db.align()
+    res = db.download()
+    assert set(int(s) for s in res.status) == {0}
+    db.free()
+    a.close()
+    b.close()
+
+
+def test_ready_guard_gives_up_and_launch_ends(oracle):
+    """Failure path of the shared-task recursion, forced from the host: with the
+    ready flags of published tasks withheld and the wait bounded at 1 ms, the
+    aligner that claims a task gives up, every aligner of the tier leaves its work
+    loop, the launch ends and the call reports BW_STATUS_INTERNAL.  The same
+    aligner then aligns the pairs bit-exactly again."""
+    _say("ready guard")
+    pairs = bw.generate_dataset(bw.CONFIGS["C3"], num_pairs=2).pairs()
+    packed = bw.pack_pairs(pairs)
+    a = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
+    a.set_ready_guard(ticks=100_000, withhold=True)
+    with pytest.raises(bw.BiwfaError, match="no progress"):
+        a.align_batch(packed)
+    a.set_ready_guard(0, False)
+    res = a.align_batch(packed)
+    ref = oracle.align(pairs, threads=16)
+    _compare(res, ref.alg_score, [ref.cigar(i) for i in range(len(pairs))], "after-guard")
+    a.close()

@@ -234,3 +234,51 @@ def test_runs_roundtrip():
     from oracle.oracle import cigar_to_runs
     for c in ("14M1I86M", "1X", "3D2I1M", ""):
         assert runs_to_cigar(cigar_to_runs(c)) == c
+
+
+def _c5_golden():
+    import gzip
+    import json
+    import os
+    from conftest import GOLDEN
+    d = os.path.join(GOLDEN, "c5")
+    with gzip.open(os.path.join(d, "c5_first4.alg.gz"), "rt") as fh:
+        tmp = [line.rstrip("\n").split("\t") for line in fh if line.strip()]
+    alg = [(int(a), b) for a, b in tmp]
+    meta = json.load(open(os.path.join(d, "c5_first4.json")))
+    return alg, meta
+
+
+def test_c5_golden_inputs_and_cigars_valid():
+    """The C5 golden vectors (first 4 pairs, 100 kbp, WFA2-lib from the reference
+    sources; tests/golden/make_c5_golden.py): the generator still yields the same
+    inputs (SHA-256), and every golden CIGAR spells its pair and rescores to its
+    score."""
+    import hashlib
+    bw = pytest.importorskip("biwfa_gpu_pii2025_amd")
+    alg, meta = _c5_golden()
+    ds = bw.generate_dataset(bw.CONFIGS["C5"], num_pairs=len(alg))
+    for i, (score, cig) in enumerate(alg):
+        p, t = ds.pair(i)
+        m = meta["pairs"][i]
+        assert hashlib.sha256(p.encode()).hexdigest() == m["pattern_sha256"], i
+        assert hashlib.sha256(t.encode()).hexdigest() == m["text_sha256"], i
+        assert m["status"] == 0
+        assert cigar_spells(p, t, cig), i
+        assert cigar_score(cig, 0, 4, 6, 2) == score, i
+        assert -m["wfa2_cigar_score"] <= -score   # breakpoint score vs the CIGAR's score
+
+
+@pytest.mark.slow
+def test_oracle_c5_real_shape_vs_wfa2_golden(oracle):
+    """The oracle on the 4 C5 golden pairs at their real shape (100 kbp, ~11
+    recursion levels, s ~ 1.9e5): score and CIGAR byte for byte against WFA2-lib.
+    About 200 s per pair on one core; run with BIWFA_RUN_SLOW=1."""
+    bw = pytest.importorskip("biwfa_gpu_pii2025_amd")
+    alg, meta = _c5_golden()
+    ds = bw.generate_dataset(bw.CONFIGS["C5"], num_pairs=len(alg))
+    r = oracle.align((ds.seq, ds.p_off, ds.p_len, ds.t_off, ds.t_len), threads=len(alg))
+    for i, (score, cig) in enumerate(alg):
+        assert int(r.alg_score[i]) == score, i
+        assert r.cigar(i) == cig, i
+        assert int(r.wf_score[i]) == -meta["pairs"][i]["wfa2_cigar_score"], i
