@@ -72,6 +72,13 @@ class _Result(ctypes.Structure):
                 ("cigar_total", ctypes.c_int64)]
 
 
+class _DevResult(ctypes.Structure):
+    """bw_device_result_t: per-pair outputs in device memory."""
+    _fields_ = [("score", ctypes.c_void_p), ("wf_score", ctypes.c_void_p),
+                ("status", ctypes.c_void_p), ("num_runs", ctypes.c_void_p),
+                ("runs", ctypes.c_void_p), ("run_offset", ctypes.c_void_p)]
+
+
 MAX_TIERS = 8
 _PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
 
@@ -135,6 +142,8 @@ def lib():
         L.bw_set_progress.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.bw_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.bw_test_set_ready_guard.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
+        L.bw_align_device_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch),
+                                              ctypes.POINTER(_DevResult)]
         L.bw_dataset_lengths.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p, ctypes.c_void_p]
         L.bw_dataset_generate.argtypes = [ctypes.POINTER(_DatasetParams), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]

@@ -19,6 +19,7 @@ constexpr int kStEndUnreachable = -3;
 constexpr int kStMaxSteps = -100;
 constexpr int kStUnattainable = -300;
 constexpr int kStCapacity = -400;
+constexpr int kStInvalid = -402;     // a malformed pair (no-allocation path's admission)
 constexpr int kStGuard = -405;       // termination guard tripped (never in a correct run)
 
 constexpr int kMaxScope = 32;          // max(o+e, x)+1 supported by the ring layout

@@ -1987,6 +1987,54 @@ __global__ void biwfa_reset(int32_t* counters, const int32_t* init, int n) {
   if (i == 0) counters[n] = (counters[n] + 1) & 0x0fffffff;
 }
 
+// No-allocation path (bw_align_device_noalloc): zeroes its two tiers' counters
+// (the admit kernel counts the LDS tier's pairs) and advances the call counter
+// at counters[n].  A kernel, not a memset, so a caller may capture the calls.
+__global__ void biwfa_noalloc_reset(int32_t* counters, int n) {
+  const int i = threadIdx.x;
+  if (i < n) counters[i] = 0;
+  if (i == 0) counters[n] = (counters[n] + 1) & 0x0fffffff;
+}
+
+// No-allocation path: admits every well-formed pair with p + t <= max_pt to the
+// LDS tier's list (in whatever order the waves reach it: results do not depend
+// on the order).  A pair with a bad length, word offset or run capacity
+// (runs_off[i+1] - runs_off[i] < p + t + 1) gets BW_STATUS_INVALID_ARG, a longer
+// one BW_STATUS_DEVICE_CAPACITY; neither is aligned.
+__global__ __launch_bounds__(256) void biwfa_noalloc_admit(KParams p, int64_t n, int64_t num_words, int max_pt,
+                                                           int32_t* list, int32_t* count) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    int st = 1;   // 1: not a pair of this call
+    if (i < n) {
+      const int pl = p.p_len[i], tl = p.t_len[i];
+      const int64_t pw = p.p_woff[i], tw = p.t_woff[i];
+      const int64_t r0 = p.runs_off[i], r1 = p.runs_off[i + 1];
+      st = 0;
+      if (pl < 0 || tl < 0 || pw < 1 || tw < 1 || pw + (pl + 31) / 32 + 2 > num_words ||
+          tw + (tl + 31) / 32 + 2 > num_words || r0 < 0 || r1 - r0 < (int64_t)pl + tl + 1)
+        st = kStInvalid;
+      else if ((int64_t)pl + tl > max_pt)
+        st = kStCapacity;
+      if (st != 0) {
+        p.status[i] = st;
+        p.score[i] = -1;
+        p.wf_score[i] = INT_MIN;
+        p.nruns[i] = 0;
+      }
+    }
+    // one atomic per wave for its admitted pairs
+    const uint64_t adm = __ballot(st == 0);
+    int first = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0 && adm) first = atomicAdd(count, __builtin_popcountll(adm));
+    first = __shfl(first, 0, kWave);
+    if (st == 0) {
+      const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(adm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)adm, 0));
+      list[first + r] = (int32_t)i;
+    }
+  }
+}
+
 // Copies each pair's runs into the compacted layout (one wave per pair).
 __global__ __launch_bounds__(256) void biwfa_compact(const uint32_t* runs, const int64_t* runs_off,
                                                      const int32_t* nruns, const int64_t* out_off,

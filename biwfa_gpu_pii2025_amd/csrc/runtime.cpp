@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -23,6 +24,9 @@
 namespace bw {
 template <class K>
 __global__ void biwfa_persistent(KParams p);
+__global__ void biwfa_noalloc_reset(int32_t* counters, int n);
+__global__ void biwfa_noalloc_admit(KParams p, int64_t n, int64_t num_words, int max_pt, int32_t* list,
+                                    int32_t* count);
 __global__ void biwfa_reset(int32_t* counters, const int32_t* init, int n);
 __global__ void biwfa_compact(const uint32_t* runs, const int64_t* runs_off, const int32_t* nruns,
                               const int64_t* out_off, uint32_t* out, int n);
@@ -221,6 +225,7 @@ struct bw_aligner {
   const int32_t* last_counters = nullptr;   // counters of the last launch sequence (d_counters or scratch)
   int64_t ready_ticks = kReadyTicksDefault; // bound of a shared-task ready-flag wait (100 MHz ticks)
   int withhold_ready = 0;                   // failure-path test hook (bw_test_set_ready_guard)
+  int32_t* scratch = nullptr;               // this device's no-allocation scratch (bw_align_device_noalloc)
   int graph = 0;                   // replay a captured launch sequence (bw_set_graph)
   bool last_graph = false;         // the last sequence was a graph replay (no per-tier events)
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
@@ -231,6 +236,38 @@ struct bw_aligner {
   std::vector<uint32_t> sam;
   int schedule = BW_SCHEDULE_LENGTH_SORTED;
 };
+
+// ---------------- scratch of the no-allocation entry point ----------------
+// bw_align_device_noalloc keeps everything it needs beyond the caller's inputs
+// and outputs in this array: the counters of its two tiers, the LDS tier's pair
+// list, the spill tier's pair list and the spill tier's workspace slabs.  Each
+// GPU holds its own copy for the life of the process; every call on a GPU
+// reuses it (calls that overlap on one GPU race).
+constexpr int64_t kNaMaxPairs = 1 << 20;          // pairs per call
+constexpr int kNaMaxPT = 2040;                    // p + t of the longest pair it aligns
+constexpr int64_t kNaScratchBytes = 64ll << 20;   // 64 MiB per GPU
+constexpr int64_t kNaCounterInts = 64;            // 2 tiers x kQctlInts + call counter, padded
+static __device__ __align__(256) int32_t scratch[kNaScratchBytes / 4];
+
+// Device address of `scratch` on each device (hipGetSymbolAddress with the
+// device current, once per device; the host-side name is not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch_ptr[kMaxDevices] = {};
+
+static hipError_t scratch_for_device(int dev, int32_t** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<int32_t*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 extern "C" {
 
@@ -305,6 +342,16 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
     set_error("stream / counter allocation failed");
     delete a;
     return nullptr;
+  }
+  {
+    const hipError_t e = scratch_for_device(a->dev, &a->scratch);
+    if (e != hipSuccess) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      (void)hipStreamDestroy(a->stream);
+      (void)hipFree(a->d_counters);
+      delete a;
+      return nullptr;
+    }
   }
   for (auto& e : a->ev) (void)hipEventCreate(&e);
   for (auto& e : a->tev) (void)hipEventCreate(&e);
@@ -1195,6 +1242,114 @@ int bw_test_set_ready_guard(bw_aligner_t* a, int64_t ready_ticks, int withhold_r
   if (!a || ready_ticks < 0) return BW_STATUS_INVALID_ARG;
   a->ready_ticks = ready_ticks > 0 ? ready_ticks : kReadyTicksDefault;
   a->withhold_ready = withhold_ready ? 1 : 0;
+  return 0;
+}
+
+// ---------------- no-allocation entry point ----------------
+
+int bw_align_device_noalloc(bw_aligner_t* a, const bw_batch_t* in, const bw_device_result_t* out) {
+  if (!a || !in || !out) {
+    set_error("bw_align_device_noalloc: null aligner, batch or result");
+    return BW_STATUS_INVALID_ARG;
+  }
+  if (in->num_pairs < 0 || in->num_pairs > kNaMaxPairs) {
+    set_error("bw_align_device_noalloc: num_pairs must be in [0, " + std::to_string(kNaMaxPairs) + "]");
+    return BW_STATUS_INVALID_ARG;
+  }
+  const int64_t n = in->num_pairs;
+  if (n > 0 && (!in->words || !in->pattern_word_offset || !in->pattern_length || !in->text_word_offset ||
+                !in->text_length || !out->score || !out->wf_score || !out->status || !out->num_runs ||
+                !out->runs || !out->run_offset)) {
+    set_error("bw_align_device_noalloc: a batch or result pointer is null");
+    return BW_STATUS_INVALID_ARG;
+  }
+  if (!a->scratch) {
+    set_error("bw_align_device_noalloc: no scratch address for this device");
+    return BW_STATUS_INTERNAL;
+  }
+  HIP_TRY(hipSetDevice(a->dev));
+  if (n == 0) return 0;
+  // Tiers: the LDS tier over every admitted pair, then one single-wave global
+  // tier (ring width 2048, the widest wavefront of any pair with p + t <= 2040)
+  // over the pairs it passes on, with its slabs in the scratch.
+  Tier t0 = make_lds_tier(a);
+  if (const int st = check_lds_layout(a, t0)) return st;
+  Tier t1 = make_global_tier(a, 0, 2048, true, kNaMaxPT, (kNaMaxPT + 31) / 32 + 3, kNaScratchBytes, 1);
+  int32_t* const counters = a->scratch;
+  int32_t* const list0 = counters + kNaCounterInts;
+  int32_t* const list1 = list0 + kNaMaxPairs;
+  int32_t* const slabs = list1 + kNaMaxPairs;
+  const int64_t slab_ints = kNaScratchBytes / 4 - (slabs - counters);
+  // as many single-wave aligners as the slabs hold (blocks of 4, or of 1 when
+  // fewer than 4 fit: penalties with a deep base-case bound need large arenas)
+  int64_t aligners = std::min<int64_t>(t1.aligners, slab_ints / t1.stride);
+  if (aligners < 1) {
+    set_error("bw_align_device_noalloc: one spill-tier workspace for these penalties exceeds the scratch");
+    return BW_STATUS_INVALID_ARG;
+  }
+  if (aligners >= bw::kWavesPerBlock) {
+    aligners -= aligners % bw::kWavesPerBlock;
+  } else {
+    t1.waves_per_block = 1;
+    t1.dyn_lds = (size_t)t1.seq_lds_words * 16;
+  }
+  t1.blocks = (int)(aligners / t1.waves_per_block);
+  t1.aligners = aligners;
+  hipStream_t s = a->stream;
+  bw::KParams p{};
+  p.words = in->words;
+  p.p_woff = in->pattern_word_offset; p.p_len = in->pattern_length;
+  p.t_woff = in->text_word_offset; p.t_len = in->text_length;
+  p.x = a->pen.x; p.o = a->pen.o; p.e = a->pen.e; p.scope = a->pen.scope;
+  p.um = a->pen.um; p.ux = a->pen.ux; p.uo = a->pen.uo; p.ue = a->pen.ue;
+  p.gmask = null_score_mask(a->pen);
+  p.score_only = a->attr.scope == BW_SCOPE_SCORE;
+  p.max_steps = a->attr.max_alignment_steps > 0 ? a->attr.max_alignment_steps : INT32_MAX;
+  p.score = out->score; p.wf_score = out->wf_score; p.status = out->status;
+  p.runs = out->runs; p.runs_off = out->run_offset; p.nruns = out->num_runs;
+  p.task = 1;
+  p.task_min_length = INT32_MAX;   // every pair depth first: no shared tasks, no item pool
+  p.ready_ticks = a->ready_ticks;
+  p.calls = counters + bw::kQctlInts * 2;
+  HIP_TRY(hipEventRecord(a->ev[0], s));
+  hipLaunchKernelGGL(bw::biwfa_noalloc_reset, dim3(1), dim3(64), 0, s, counters, bw::kQctlInts * 2);
+  HIP_TRY(hipGetLastError());
+  {
+    const int threads = 256;
+    const int blocks = (int)std::min<int64_t>((n + threads - 1) / threads, 4096);
+    hipLaunchKernelGGL(bw::biwfa_noalloc_admit, dim3(blocks), dim3(threads), 0, s, p, n, in->num_words,
+                       kNaMaxPT, list0, counters + 1);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(a->ev[1], s));
+  HIP_TRY(hipEventRecord(a->tev[0], s));
+  const Tier* tiers[2] = {&t0, &t1};
+  for (int t = 0; t < 2; ++t) {
+    const Tier& T = *tiers[t];
+    bw::KParams q = p;
+    q.list = t == 0 ? list0 : list1;
+    q.qctl = counters + bw::kQctlInts * t;
+    q.queue = q.qctl;
+    q.count = q.qctl + 1;
+    q.epoch = t;
+    q.chunk = T.chunk;
+    q.next_list = t == 0 ? list1 : nullptr;
+    q.next_count = t == 0 ? counters + bw::kQctlInts + 1 : nullptr;
+    q.L = T.L;
+    q.gws = slabs;
+    q.gws_stride = T.stride;
+    q.seq_lds_words = T.seq_lds_words;
+    void* args[] = {&q};
+    const void* kern = T.lds ? lds_kernel_for(T.L.wcap, default_penalties(a->pen)) : team_kernel(1);
+    HIP_TRY(hipLaunchKernel(kern, dim3(T.blocks), dim3(bw::kWave * T.waves_per_block), args, T.dyn_lds, s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(a->tev[t + 1], s));
+    if (t == 0) HIP_TRY(hipEventRecord(a->ev[2], s));
+  }
+  HIP_TRY(hipEventRecord(a->ev[3], s));
+  a->last_counters = counters;
+  a->last_ntiers = 2;
+  a->last_graph = false;
   return 0;
 }
 

@@ -269,6 +269,37 @@ int bw_set_graph(bw_aligner_t* aligner, int enable);
  * BW_STATUS_INTERNAL.  Affects later bw_dbatch_align / bw_align_batch calls. */
 int bw_test_set_ready_guard(bw_aligner_t* aligner, int64_t ready_ticks, int withhold_ready);
 
+/* ---------------- no device allocation ---------------- */
+
+/* Per-pair results of bw_align_device_noalloc, all in device memory and owned by
+ * the caller.  Pair i's runs (encoding above, in order) are written to
+ * runs[run_offset[i] ...]; run_offset[i+1] - run_offset[i] must be at least
+ * p_i + t_i + 1.  num_runs[i] receives their count. */
+typedef struct {
+  int32_t* score;
+  int32_t* wf_score;
+  int32_t* status;
+  int32_t* num_runs;
+  uint32_t* runs;
+  const int64_t* run_offset;   /* [num_pairs + 1] */
+} bw_device_result_t;
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the variant of bw_dbatch_align that calls no hipMalloc,
+ * hipFree or allocating library routine.  `batch` holds DEVICE pointers (same
+ * layout as for bw_dbatch_upload); the results go straight to the caller's
+ * device buffers.  Everything else lives in a 64 MiB per-GPU scratch array that
+ * the library holds until the process ends.  Limits: at most 1,048,576 pairs
+ * per call (more: BW_STATUS_INVALID_ARG, nothing enqueued); pairs with
+ * p + t <= 2040 are aligned exactly as bw_dbatch_align aligns them (LDS tier,
+ * then a single-wave global tier), longer ones get status
+ * BW_STATUS_DEVICE_CAPACITY and malformed ones BW_STATUS_INVALID_ARG.
+ * Enqueued on the aligner's stream; call bw_synchronize before reading the
+ * results.  Every call on one GPU shares its scratch: calls that overlap on one
+ * GPU (other streams or threads) race. */
+int bw_align_device_noalloc(bw_aligner_t* aligner, const bw_batch_t* batch, const bw_device_result_t* result);
+
 /* ---------------- (v) learned cost model ---------------- */
 
 /* No counterpart in the reference: a 4 -> 64 -> 64 -> 1 ReLU MLP (BF16 weights,
