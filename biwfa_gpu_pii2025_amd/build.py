@@ -8,6 +8,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -45,19 +46,30 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build(verbose=False):
+# biwfa_kernel.hip is compiled once per group of kernel instantiations
+# (-DBW_PART=i), so the groups build in parallel.
+KERNEL_PARTS = 8
+
+
+def build(verbose=False, jobs=None):
     """Compiles every source (always: this is the does-it-build check)."""
     os.makedirs(OBJ, exist_ok=True)
     hipcc = _hipcc()
-    objs = []
-    for src in ("biwfa_kernel.hip", "runtime.cpp", "costmodel.hip"):
+    steps = []
+    for part in range(KERNEL_PARTS):
+        obj = os.path.join(OBJ, "biwfa_kernel_%02d.o" % part)
+        steps.append(([hipcc] + HIP_FLAGS + ["-DBW_PART=%d" % part, "-c", os.path.join(CSRC, "biwfa_kernel.hip"),
+                                            "-o", obj], obj))
+    for src in ("runtime.cpp", "costmodel.hip"):
         obj = os.path.join(OBJ, src.split(".")[0] + ".o")
-        _run([hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        steps.append(([hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
     for src in ("dataset.cpp",):
         obj = os.path.join(OBJ, src.split(".")[0] + ".o")
-        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        steps.append((["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
+    jobs = jobs or max(1, min(len(steps), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        list(pool.map(lambda st: _run(st[0], verbose), steps))
+    objs = [obj for _, obj in steps]
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp"], verbose)
     os.replace(tmp, LIB)

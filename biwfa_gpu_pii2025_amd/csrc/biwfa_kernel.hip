@@ -117,28 +117,12 @@ DI int wave_min_any(int v) {
 }
 
 // A wavefront component as seen by readers: offsets for k in [lo,hi] live at
-// ws[base + (k & mask)] (mask = wcap-1 in a ring, -1 in the arena).  lo > hi is
-// the null wavefront (WFA2's wavefront_null has lo=1, hi=-1).
+// element base + (k & mask) of the offset store (mask = wcap-1 in a ring, -1 in
+// the arena).  lo > hi is the null wavefront (WFA2's wavefront_null has lo=1, hi=-1).
 struct Wf {
   int lo, hi, base, mask;
 };
 DI Wf wf_null() { return Wf{1, -1, 0, 0}; }
-DI int rd(const int32_t* ws, const Wf& w, int k) {
-  return (k >= w.lo && k <= w.hi) ? ws[(uint32_t)(w.base + (k & w.mask))] : kOffsetNull;
-}
-// rd without a branch: every lane loads, at k clamped into [lo, hi] (for the
-// null wavefront lo=1, hi=-1, mask 0: index base+0), and lanes outside take NULL.
-DI int rd_clamped(const int32_t* ws, const Wf& w, int k) {
-  const int kk = imin(imax(k, w.lo), w.hi);
-  const int v = ws[(uint32_t)(w.base + (kk & w.mask))];
-  return ((k >= w.lo) & (k <= w.hi)) ? v : kOffsetNull;
-}
-// rd without a branch for a ring input: base + (k & mask) stays inside the
-// ring slot for every k, so no clamp is needed.
-DI int rd_ring(const int32_t* ws, const Wf& w, int k) {
-  const int v = ws[(uint32_t)(w.base + (k & w.mask))];
-  return ((k >= w.lo) & (k <= w.hi)) ? v : kOffsetNull;
-}
 
 // One aligner's view of a sub-problem.
 struct Dir {
@@ -156,7 +140,8 @@ struct BP {                 // wavefront_bialigner.h:43-54
 
 template <class K>
 struct Ctx {
-  int32_t* ws;    // shared workspace (ring / arena)
+  int32_t* ws;    // shared workspace (ring / arena, then the private regions)
+  int32_t* wd;    // the ring / arena offsets (at the workspace base)
   int32_t* pv;    // this wave's private region (base headers, backtrace runs, stack)
   const uint64_t* P;
   const uint64_t* T;
@@ -187,6 +172,39 @@ struct Ctx {
   int guard;      // score bound of the current sub-problem (termination guard)
   int epoch;      // ready-flag value of this launch: call count << 3 | tier (never 0)
 };
+
+// ---------------------------------------------------------------------------
+// offset store: WFA2's int32 offsets, indexed from the start of the ring /
+// arena region (vector accessors load or store 4 consecutive offsets).
+template <class K>
+DI int ld_off(const Ctx<K>& c, int i) { return c.wd[(uint32_t)i]; }
+template <class K>
+DI void st_off(const Ctx<K>& c, int i, int v) { c.wd[(uint32_t)i] = v; }
+// 4 offsets at a multiple-of-4 index (16-byte aligned: ring slots start at
+// multiples of the ring width, global-tier arena wavefronts at multiples of 4)
+template <class K>
+DI int4 ld_off4(const Ctx<K>& c, int i) { return *reinterpret_cast<const int4*>(c.wd + (uint32_t)i); }
+template <class K>
+DI void st_off4(const Ctx<K>& c, int i, const int4& v) { *reinterpret_cast<int4*>(c.wd + (uint32_t)i) = v; }
+template <class K>
+DI int rd(const Ctx<K>& c, const Wf& w, int k) {
+  return (k >= w.lo && k <= w.hi) ? ld_off(c, w.base + (k & w.mask)) : kOffsetNull;
+}
+// rd without a branch: every lane loads, at k clamped into [lo, hi] (for the
+// null wavefront lo=1, hi=-1, mask 0: index base+0), and lanes outside take NULL.
+template <class K>
+DI int rd_clamped(const Ctx<K>& c, const Wf& w, int k) {
+  const int kk = imin(imax(k, w.lo), w.hi);
+  const int v = ld_off(c, w.base + (kk & w.mask));
+  return ((k >= w.lo) & (k <= w.hi)) ? v : kOffsetNull;
+}
+// rd without a branch for a ring input: base + (k & mask) stays inside the
+// ring slot for every k, so no clamp is needed.
+template <class K>
+DI int rd_ring(const Ctx<K>& c, const Wf& w, int k) {
+  const int v = ld_off(c, w.base + (k & w.mask));
+  return ((k >= w.lo) & (k <= w.hi)) ? v : kOffsetNull;
+}
 
 // Upper bound of the optimal score of a plen x tlen sub-problem plus the
 // slack the breakpoint search may run past it; a search that goes beyond it
@@ -277,12 +295,12 @@ DI int team_min_slot(const Ctx<K>& c, const int32_t* buf, int j) {
 // ---------------------------------------------------------------------------
 // headers
 
-// Start of the ring / base-case arena in the workspace: offset 0 by
-// construction, a compile-time constant in the LDS kernel (every uniform value
-// the LDS kernel keeps out of its scalar registers helps: it runs close to the
-// scalar register limit).
+// Start of the ring / base-case arena, as an index into the offset store: the
+// layout puts the offsets at the workspace base (the host checks L.data == 0),
+// so it is the compile-time constant 0 (every uniform value the kernels keep out
+// of their scalar registers helps: they run close to the scalar register limit).
 template <class K>
-DI int ws_data(const Ctx<K>& c) { return K::kLds ? 0 : c.L.data; }
+DI int ws_data(const Ctx<K>&) { return 0; }
 // ring width (a power of two): compile-time in the default LDS configuration
 template <class K>
 DI int wcap(const Ctx<K>& c) { return K::kW > 0 ? K::kW : c.L.wcap; }
@@ -452,6 +470,27 @@ DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h) {
   return imin(n, lim);
 }
 
+// Extension of two diagonals of a lane (the global tiers' vector steps call it
+// for diagonals 0-1 and 2-3 of their four): the first 32-base window of both is
+// loaded before either is compared, so their sequence reads overlap.  Returns
+// the matches within the first window (32 when it matches completely); the
+// caller continues those rare diagonals word by word.  (v, h) must be
+// in-bounds positions (callers pass 0, 0 for diagonals not extended).
+template <class Ptr>
+DI void lcp_first2(Ptr P, Ptr T, const Dir& d, int v0, int h0, int v1, int h1, int& n0, int& n1) {
+  const bool rev = d.rev != 0;
+  const int pp0 = rev ? d.pb + d.plen - 32 - v0 : d.pb + v0;
+  const int tp0 = rev ? d.tb + d.tlen - 32 - h0 : d.tb + h0;
+  const int pp1 = rev ? d.pb + d.plen - 32 - v1 : d.pb + v1;
+  const int tp1 = rev ? d.tb + d.tlen - 32 - h1 : d.tb + h1;
+  const uint64_t a0 = P[pp0 >> 5], a1 = P[(pp0 >> 5) + 1], b0 = T[tp0 >> 5], b1 = T[(tp0 >> 5) + 1];
+  const uint64_t c0 = P[pp1 >> 5], c1 = P[(pp1 >> 5) + 1], e0 = T[tp1 >> 5], e1 = T[(tp1 >> 5) + 1];
+  const uint64_t x0 = window2(a0, a1, pp0 & 31) ^ window2(b0, b1, tp0 & 31);
+  const uint64_t x1 = window2(c0, c1, pp1 & 31) ^ window2(e0, e1, tp1 & 31);
+  n0 = x0 == 0 ? 32 : (rev ? __builtin_clzll(x0) : __builtin_ctzll(x0)) >> 1;
+  n1 = x1 == 0 ? 32 : (rev ? __builtin_clzll(x1) : __builtin_ctzll(x1)) >> 1;
+}
+
 // ---------------------------------------------------------------------------
 // compute: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
 // `slot` is score % scope for ring aligners.
@@ -564,6 +603,165 @@ DI bool gather_inputs(const Ctx<K>& c, const Dir& d, int s, int slot, Wf& mm, Wf
   return ne != 0;
 }
 
+// Lane i receives lane i-1's value, lane 0 `first` (DPP wave_shr:1).
+DI int from_prev_lane(int v, int first) { return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xf, 0xf, false); }
+// Lane i receives lane i+1's value, lane 63 `last` (DPP wave_shl:1).
+DI int from_next_lane(int v, int last) { return __builtin_amdgcn_update_dpp(last, v, 0x130, 0xf, 0xf, false); }
+DI int v4get(const int4& v, int j) { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
+// a[j] for a lane-varying j, as masking arithmetic (a select of register-array
+// elements would become a select of their addresses and pin the array to scratch)
+DI int pick4(const int (&a)[4], int j) {
+  return (a[0] & -(int)(j == 0)) | (a[1] & -(int)(j == 1)) | (a[2] & -(int)(j == 2)) | (a[3] & -(int)(j == 3));
+}
+
+// Per-lane results of a step's diagonals: first/last in-bounds diagonal of
+// M, I1, D1 (trims), maximum antidiagonal, and the end component's offset at
+// the end diagonal (INT_MIN unless this lane holds it).
+struct StepAcc {
+  int fm, lm, fi, li, fd, ld, lmax, endv;
+};
+
+// Index of the 4 offsets of input w at diagonals k0 .. k0+3 (k0 a multiple of
+// 4): in a ring they stay in the slot for any k0; in the arena k0 is clamped
+// into the input's allocation (whole 4-offset groups from lo & ~3 to hi | 3),
+// and a null input reads element 0.  Values outside [lo, hi] are masked later.
+DI int in4_index(const Wf& w, int k0, bool ring) {
+  if (ring) return w.base + (k0 & w.mask);
+  if (w.lo > w.hi) return 0;
+  return w.base + imin(imax(k0, w.lo & ~3), w.hi & ~3);
+}
+DI int4 mask4(int4 v, const Wf& w, int k0) {
+  v.x = (k0 >= w.lo && k0 <= w.hi) ? v.x : kOffsetNull;
+  v.y = (k0 + 1 >= w.lo && k0 + 1 <= w.hi) ? v.y : kOffsetNull;
+  v.z = (k0 + 2 >= w.lo && k0 + 2 <= w.hi) ? v.z : kOffsetNull;
+  v.w = (k0 + 3 >= w.lo && k0 + 3 <= w.hi) ? v.w : kOffsetNull;
+  return v;
+}
+
+// A global-tier step over [lo, hi] in tiles of 256 diagonals: lane l of the
+// wave that owns a tile computes diagonals kt + 4l .. kt + 4l + 3
+// (wavefront_compute_affine_idm, wavefront_compute_affine.c:65-85, then the
+// extension) from one 16-byte read per input, with the diagonal-minus-one and
+// diagonal-plus-one inputs of its outer diagonals taken from the neighbouring
+// lanes (DPP wave shifts) and, at the tile edges, from one read each.  Tiles
+// start at lo & ~3 and go to the team's waves in turn.  Same results as the
+// one-diagonal-per-lane loop: identical arithmetic per diagonal; reads outside
+// an input's [lo, hi] are NULL; writes cover whole 4-offset groups, so the
+// caller guarantees they alias nothing live (ring: (hi|3) - (lo&~3) < wcap;
+// arena: the output allocation spans those groups).
+// A global-tier step over [lo, hi] in tiles of 256 diagonals: lane l of the
+// wave that owns a tile computes diagonals kt + 4l .. kt + 4l + 3
+// (wavefront_compute_affine_idm, wavefront_compute_affine.c:65-85, then the
+// extension) from one 16-byte read per input, with the diagonal-minus-one and
+// diagonal-plus-one inputs of its outer diagonals taken from the neighbouring
+// lanes (DPP wave shifts) and, at the tile edges, from one read each.  Tiles
+// start at lo & ~3 and go to the team's waves in turn.  Same results as the
+// one-diagonal-per-lane loop: identical arithmetic per diagonal; reads outside
+// an input's [lo, hi] are NULL; writes cover whole 4-offset groups, so the
+// caller guarantees they alias nothing live (ring: (hi|3) - (lo&~3) < wcap;
+// arena: the output allocation spans those groups).
+template <class K, bool kMax>
+DI void step_tiles(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, const Wf& mo, const Wf& ie,
+                   const Wf& de, bool has_i, bool has_d, int bm, int bi, int bd, int mask, StepAcc& acc) {
+  const bool ring = d.ring >= 0;
+  const uint32_t tl = (uint32_t)d.tlen, pl = (uint32_t)d.plen;
+  const int k_end = d.tlen - d.plen;
+  const int ce = d.ce;
+  const int t0 = lo & ~3;
+  constexpr int kTile = 4 * kWave;
+  for (int kt = t0 + c.wid * kTile; kt <= hi; kt += K::kNW * kTile) {
+    const int k0 = kt + 4 * c.lane;
+    // every input read issued before any is used
+    int4 vm = ld_off4(c, in4_index(mm, k0, ring));
+    int4 vo = ld_off4(c, in4_index(mo, k0, ring));
+    int4 vi = ld_off4(c, in4_index(ie, k0, ring));
+    int4 vd = ld_off4(c, in4_index(de, k0, ring));
+    const int o_prev = ring ? rd_ring(c, mo, kt - 1) : rd_clamped(c, mo, kt - 1);
+    const int i_prev = ring ? rd_ring(c, ie, kt - 1) : rd_clamped(c, ie, kt - 1);
+    const int o_next = ring ? rd_ring(c, mo, kt + kTile) : rd_clamped(c, mo, kt + kTile);
+    const int d_next = ring ? rd_ring(c, de, kt + kTile) : rd_clamped(c, de, kt + kTile);
+    // a tile strictly inside every input needs no masks (a null input is never inside)
+    const bool inner = mm.lo <= kt && kt + kTile - 1 <= mm.hi && mo.lo <= kt - 1 && kt + kTile <= mo.hi &&
+                       ie.lo <= kt - 1 && kt + kTile - 2 <= ie.hi && de.lo <= kt + 1 && kt + kTile <= de.hi;
+    if (!inner) {
+      vm = mask4(vm, mm, k0);
+      vo = mask4(vo, mo, k0);
+      vi = mask4(vi, ie, k0);
+      vd = mask4(vd, de, k0);
+    }
+    // neighbours across lanes (all lanes active here)
+    const int o_m1 = from_prev_lane(vo.w, o_prev), i_m1 = from_prev_lane(vi.w, i_prev);
+    const int o_p1 = from_next_lane(vo.x, o_next), d_p1 = from_next_lane(vd.x, d_next);
+    int ins1[4], del1[4], mx[4], vv[4], hh[4], ext[4];
+    bool act[4], live[4];
+    ins1[0] = imax(o_m1, i_m1) + 1;
+    ins1[1] = imax(vo.x, vi.x) + 1;
+    ins1[2] = imax(vo.y, vi.y) + 1;
+    ins1[3] = imax(vo.z, vi.z) + 1;
+    del1[0] = imax(vo.y, vd.y);
+    del1[1] = imax(vo.z, vd.z);
+    del1[2] = imax(vo.w, vd.w);
+    del1[3] = imax(o_p1, d_p1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + j;
+      mx[j] = imax(del1[j], imax(v4get(vm, j) + 1, ins1[j]));
+      act[j] = k >= lo && k <= hi;
+      live[j] = act[j] && !((uint32_t)mx[j] > tl || (uint32_t)(mx[j] - k) > pl);
+      vv[j] = live[j] ? mx[j] - k : 0;
+      hh[j] = live[j] ? mx[j] : 0;
+    }
+    if (c.seq_lds) {
+      lcp_first2((lds_words)c.P, (lds_words)c.T, d, vv[0], hh[0], vv[1], hh[1], ext[0], ext[1]);
+      lcp_first2((lds_words)c.P, (lds_words)c.T, d, vv[2], hh[2], vv[3], hh[3], ext[2], ext[3]);
+    } else {
+      lcp_first2((global_words)c.P, (global_words)c.T, d, vv[0], hh[0], vv[1], hh[1], ext[0], ext[1]);
+      lcp_first2((global_words)c.P, (global_words)c.T, d, vv[2], hh[2], vv[3], hh[3], ext[2], ext[3]);
+    }
+    // a diagonal whose first window matched completely continues word by word
+    // (rare away from the alignment path); one diagonal at a time per lane
+    int more = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      more |= (live[j] && ext[j] == 32 && imin(d.plen - vv[j], d.tlen - hh[j]) > 32) ? (1 << j) : 0;
+    while (more) {
+      const int j = __builtin_ctz(more);
+      more &= more - 1;
+      const int v = pick4(vv, j), h = pick4(hh, j);
+      const int r = 32 + lcp(c, d, v + 32, h + 32);
+      ext[0] = j == 0 ? r : ext[0];
+      ext[1] = j == 1 ? r : ext[1];
+      ext[2] = j == 2 ? r : ext[2];
+      ext[3] = j == 3 ? r : ext[3];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ext[j] = imin(ext[j], imin(d.plen - vv[j], d.tlen - hh[j]));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + j;
+      if (live[j]) {
+        mx[j] += ext[j];
+        if constexpr (kMax) acc.lmax = imax(acc.lmax, 2 * mx[j] - k);
+      } else {
+        mx[j] = kOffsetNull;
+      }
+      // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
+      const bool in_i = act[j] && (uint32_t)ins1[j] <= tl && (uint32_t)(ins1[j] - k) <= pl;
+      const bool in_d = act[j] && (uint32_t)del1[j] <= tl && (uint32_t)(del1[j] - k) <= pl;
+      acc.fm = imin(acc.fm, live[j] ? k : INT_MAX); acc.lm = imax(acc.lm, live[j] ? k : INT_MIN);
+      acc.fi = imin(acc.fi, in_i ? k : INT_MAX); acc.li = imax(acc.li, in_i ? k : INT_MIN);
+      acc.fd = imin(acc.fd, in_d ? k : INT_MAX); acc.ld = imax(acc.ld, in_d ? k : INT_MIN);
+      if (k == k_end && act[j]) acc.endv = (ce == CM) ? mx[j] : (ce == CI) ? ins1[j] : del1[j];
+    }
+    if (k0 <= hi) {   // whole groups: positions past hi (up to hi | 3) are never read
+      const int o = ring ? (k0 & mask) : k0;
+      st_off4(c, bm + o, int4{mx[0], mx[1], mx[2], mx[3]});
+      if (has_i) st_off4(c, bi + o, int4{ins1[0], ins1[1], ins1[2], ins1[3]});
+      if (has_d) st_off4(c, bd + o, int4{del1[0], del1[1], del1[2], del1[3]});
+    }
+  }
+}
+
 // One WFA step at score s: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
 // followed by wavefront_extend_end2end(_max) (wavefront_extend.c:86-211) and the
 // termination test (wavefront_termination.c:37-114), fused: the lane that
@@ -610,7 +808,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     bi = ring_base(c, d.ring, slot, CI);
     bd = ring_base(c, d.ring, slot, CD);
     mask = wcap(c) - 1;
-  } else {
+  } else if constexpr (K::kLds) {
     const int need = width * (1 + (int)has_i + (int)has_d);
     if (c.arena_top + need > arena_cap(c)) { c.spill = 1; return 0; }
     int top = ws_data(c) + c.arena_top;
@@ -619,27 +817,42 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     bd = top - lo; if (has_d) top += width;
     c.arena_top = top - ws_data(c);
     mask = -1;
+  } else {
+    // global tiers: each component spans whole 4-offset groups (lo & ~3 to
+    // hi | 3) from a multiple-of-4 index, for the 16-byte reads and writes
+    const int a0 = lo & ~3, span = (hi | 3) - a0 + 1;
+    int top = (c.arena_top + 3) & ~3;
+    const int need = span * (1 + (int)has_i + (int)has_d);
+    if (top + need > arena_cap(c)) { c.spill = 1; return 0; }
+    bm = top - a0; top += span;
+    bi = top - a0; if (has_i) top += span;
+    bd = top - a0; if (has_d) top += span;
+    c.arena_top = top;
+    mask = -1;
   }
   const uint32_t tl = (uint32_t)d.tlen, pl = (uint32_t)d.plen;
   const int k_end = d.tlen - d.plen;
   const int ce = d.ce;
   int fm = INT_MAX, lm = INT_MIN, fi = INT_MAX, li = INT_MIN, fd = INT_MAX, ld = INT_MIN;
   int lmax = 0, endv = INT_MIN, endu = INT_MIN;
-  int32_t* const ws = c.ws;
-  if (K::kLds && width <= kWave) {
-    // One chunk holds the whole wavefront (always, for ring widths up to 64):
-    // no loop, branch-free input reads, trims straight from the ballots.
+  if (width <= kWave) {
+    // One chunk holds the whole wavefront (in the LDS tier always, for ring
+    // widths up to 64; elsewhere the narrow steps of deep recursion levels and
+    // base cases): no loop, branch-free input reads, trims straight from the
+    // ballots.  In a team only wave 0 works; the others add neutral values to
+    // the team reduction.
+    if (K::kNW == 1 || c.wid == 0) {
     const int k = lo + c.lane;
     const bool act = k <= hi;
     int ins1, del1, mx;
     if (d.ring >= 0) {
-      ins1 = imax(rd_ring(ws, mo, k - 1), rd_ring(ws, ie, k - 1)) + 1;
-      del1 = imax(rd_ring(ws, mo, k + 1), rd_ring(ws, de, k + 1));
-      mx = imax(del1, imax(rd_ring(ws, mm, k) + 1, ins1));
+      ins1 = imax(rd_ring(c, mo, k - 1), rd_ring(c, ie, k - 1)) + 1;
+      del1 = imax(rd_ring(c, mo, k + 1), rd_ring(c, de, k + 1));
+      mx = imax(del1, imax(rd_ring(c, mm, k) + 1, ins1));
     } else {
-      ins1 = imax(rd_clamped(ws, mo, k - 1), rd_clamped(ws, ie, k - 1)) + 1;
-      del1 = imax(rd_clamped(ws, mo, k + 1), rd_clamped(ws, de, k + 1));
-      mx = imax(del1, imax(rd_clamped(ws, mm, k) + 1, ins1));
+      ins1 = imax(rd_clamped(c, mo, k - 1), rd_clamped(c, ie, k - 1)) + 1;
+      del1 = imax(rd_clamped(c, mo, k + 1), rd_clamped(c, de, k + 1));
+      mx = imax(del1, imax(rd_clamped(c, mm, k) + 1, ins1));
     }
     const bool live = act && !((uint32_t)mx > tl || (uint32_t)(mx - k) > pl);
     int ak = 0;
@@ -649,9 +862,9 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     }
     mx = live ? mx : kOffsetNull;
     if (act) {
-      ws[bm + (k & mask)] = mx;
-      if (has_i) ws[bi + (k & mask)] = ins1;
-      if (has_d) ws[bd + (k & mask)] = del1;
+      st_off(c, bm + (k & mask), mx);
+      if (has_i) st_off(c, bi + (k & mask), ins1);
+      if (has_d) st_off(c, bd + (k & mask), del1);
     }
     const uint64_t b_m = __ballot(live);
     const uint64_t b_i = __ballot(act && (uint32_t)ins1 <= tl && (uint32_t)(ins1 - k) <= pl);
@@ -662,6 +875,16 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     const int ev = (ce == CM) ? mx : (ce == CI) ? ins1 : del1;
     if (k_end >= lo && k_end <= hi) endu = rdl(ev, k_end - lo);
     if constexpr (kMax) lmax = wave_max_nonneg(ak);
+    }
+  } else if (!K::kLds && (d.ring < 0 || (hi | 3) - (lo & ~3) + 1 <= wcap(c))) {
+    StepAcc acc{INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, 0, INT_MIN};
+    step_tiles<K, kMax>(c, d, lo, hi, mm, mo, ie, de, has_i, has_d, bm, bi, bd, mask, acc);
+    fm = wave_min_any(acc.fm); lm = wave_max_any(acc.lm);
+    fi = wave_min_any(acc.fi); li = wave_max_any(acc.li);
+    fd = wave_min_any(acc.fd); ld = wave_max_any(acc.ld);
+    // offset at k_end: lane ((k_end - (lo & ~3)) / 4) % 64 of the wave owning its tile
+    if (k_end >= lo && k_end <= hi) endu = rdl(acc.endv, ((k_end - (lo & ~3)) >> 2) & (kWave - 1));
+    if constexpr (kMax) lmax = wave_max_nonneg(acc.lmax);
   } else {
   // A wave takes kG of its chunks at a time and issues all their loads before
   // any store: the stores of one chunk could alias the next chunk's inputs as
@@ -676,9 +899,9 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     for (int g = 0; g < kG; ++g) {
       const int k = kb0 + g * kStride + c.lane;
       // wavefront_compute_affine_idm (wavefront_compute_affine.c:65-85)
-      ins1[g] = imax(rd(ws, mo, k - 1), rd(ws, ie, k - 1)) + 1;
-      del1[g] = imax(rd(ws, mo, k + 1), rd(ws, de, k + 1));
-      misms[g] = rd(ws, mm, k) + 1;
+      ins1[g] = imax(rd(c, mo, k - 1), rd(c, ie, k - 1)) + 1;
+      del1[g] = imax(rd(c, mo, k + 1), rd(c, de, k + 1));
+      misms[g] = rd(c, mm, k) + 1;
     }
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
@@ -695,9 +918,9 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
         mx = kOffsetNull;
       }
       if (act) {
-        ws[(uint32_t)(bm + (k & mask))] = mx;
-        if (has_i) ws[(uint32_t)(bi + (k & mask))] = ins1[g];
-        if (has_d) ws[(uint32_t)(bd + (k & mask))] = del1[g];
+        st_off(c, bm + (k & mask), mx);
+        if (has_i) st_off(c, bi + (k & mask), ins1[g]);
+        if (has_d) st_off(c, bd + (k & mask), del1[g]);
         if (k == k_end) endv = (ce == CM) ? mx : (ce == CI) ? ins1[g] : del1[g];
       }
       // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
@@ -762,7 +985,7 @@ DI bool terminated(const Ctx<K>& c, const Dir& d, int s, int slot) {
   int present;
   const Wf w = get_wf(c, d, s, slot, d.ce, &present);
   if (!present || w.lo > k_end || k_end > w.hi) return false;
-  return uni(c.ws[w.base + (k_end & w.mask)]) >= d.tlen;
+  return uni(ld_off(c, w.base + (k_end & w.mask))) >= d.tlen;
 }
 
 // Returns 1 when the aligner is done; *max_ak gets the max antidiagonal.
@@ -784,10 +1007,10 @@ DI int extend_step(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
     const int k = kb + c.lane;
     if (k <= m.hi) {
       const int a = m.base + (k & m.mask);
-      int off = c.ws[a];
+      int off = ld_off(c, a);
       if (off != kOffsetNull) {
         off += lcp(c, d, off - k, off);
-        c.ws[a] = off;
+        st_off(c, a, off);
         lmax = imax(lmax, 2 * off - k);
       }
     }
@@ -827,8 +1050,8 @@ DI void scan_breakpoint(const Ctx<K>& c, bool forward, int s0, int s1, const Wf&
     bool ok = false;
     int o0 = 0, o1 = 0;
     if (k0 <= min_hi) {
-      o0 = c.ws[w0.base + (k0 & w0.mask)];
-      o1 = c.ws[w1.base + (k1 & w1.mask)];
+      o0 = ld_off(c, w0.base + (k0 & w0.mask));
+      o1 = ld_off(c, w1.base + (k1 & w1.mask));
       ok = o0 + o1 >= tl;
       if (ok && comp != CM) {   // indel: forward coordinates must be in bounds (:226-240)
         ok = forward ? !(o0 - k0 > pl || o0 > tl) : !(o1 - k1 > pl || o1 > tl);
@@ -863,8 +1086,8 @@ DI int scan_first(const Ctx<K>& c, bool forward, const Wf& w0, const Wf& w1, int
     const int k1 = tl - pl - k0;
     bool ok = false;
     if (k0 <= min_hi) {
-      const int o0 = c.ws[w0.base + (k0 & w0.mask)];
-      const int o1 = c.ws[w1.base + (k1 & w1.mask)];
+      const int o0 = ld_off(c, w0.base + (k0 & w0.mask));
+      const int o1 = ld_off(c, w1.base + (k1 & w1.mask));
       ok = o0 + o1 >= tl;
       if (ok && comp != CM) ok = forward ? !(o0 - k0 > pl || o0 > tl) : !(o1 - k1 > pl || o1 > tl);
     }
@@ -883,8 +1106,8 @@ DI void accept_hit(const Ctx<K>& c, const int32_t* buf, int j, bool forward, int
   const int kk0 = team_min_slot(c, buf, j);
   if (kk0 == INT_MAX) return;
   const int kk1 = tl - pl - kk0;
-  const int a0 = uni(c.ws[w0.base + (kk0 & w0.mask)]);
-  const int a1 = uni(c.ws[w1.base + (kk1 & w1.mask)]);
+  const int a0 = uni(ld_off(c, w0.base + (kk0 & w0.mask)));
+  const int a1 = uni(ld_off(c, w1.base + (kk1 & w1.mask)));
   if (forward) {
     bp.sf = s0; bp.sr = s1; bp.kf = kk0; bp.kr = kk1; bp.of = a0; bp.orr = a1;
   } else {
@@ -962,8 +1185,8 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
       if (k0 <= (e0 >> 16)) {
         const int k1 = dk - k0;
         const int pp = (e1 >> 24) & 63;
-        const int o0 = c.ws[(e1 & 0xfff) + (k0 & wm)];
-        const int o1 = c.ws[((e1 >> 12) & 0xfff) + (k1 & wm)];
+        const int o0 = ld_off(c, (e1 & 0xfff) + (k0 & wm));
+        const int o1 = ld_off(c, ((e1 >> 12) & 0xfff) + (k1 & wm));
         const int ii = lane_div3(pp);
         const bool indel = pp - 3 * ii != 2;
         bool hit = o0 + o1 >= tl;
@@ -984,8 +1207,8 @@ DI void overlap_lanes(Ctx<K>& c, const Dir& a0, int slot0, const Dir& a1, int s0
   const int ccw = pp - 3 * ii;
   const int s1w = s1 - ii;
   const int kk0 = rdl(klo, pp) + (key & 63), kk1 = dk - kk0;
-  const int a0v = uni(c.ws[rdl(b0, pp) + (kk0 & wm)]);
-  const int a1v = uni(c.ws[rdl(b1, pp) + (kk1 & wm)]);
+  const int a0v = uni(ld_off(c, rdl(b0, pp) + (kk0 & wm)));
+  const int a1v = uni(ld_off(c, rdl(b1, pp) + (kk1 & wm)));
   if (forward) {
     bp.sf = s0; bp.sr = s1w; bp.kf = kk0; bp.kr = kk1; bp.of = a0v; bp.orr = a1v;
   } else {
@@ -1092,7 +1315,7 @@ DI void ring_init(Ctx<K>& c, Dir& d) {
     for (int q = 0; q < 3; ++q) c.rlo[q] = mine ? kAbsent : c.rlo[q];
   }
   set_hdr(c, d, 0, 0, d.cb, 0, 0, 0);
-  c.ws[ring_base(c, d.ring, 0, d.cb)] = 0;
+  st_off(c, ring_base(c, d.ring, 0, d.cb), 0);
   d.null_steps = 0;
   d.status = kStOk;
   d.status_score = 0;
@@ -1218,7 +1441,7 @@ DI long long bt_src(const Ctx<K>& c, const Dir& b, int score, int comp, int k, i
   int present;
   const Wf w = get_wf(c, b, score, 0, comp, &present);
   if (!present || !(w.lo <= k && k <= w.hi)) return (long long)kOffsetNull;
-  return ((long long)(uni(c.ws[w.base + k]) + add) << 4) | type;
+  return ((long long)(uni(ld_off(c, w.base + k)) + add) << 4) | type;
 }
 DI long long max64(long long a, long long b) { return a >= b ? a : b; }
 
@@ -1248,7 +1471,7 @@ DI long long bt_max_mem(const Ctx<K>& c, int score, int matrix, int k) {
   for (int j = 0; j < 5; ++j) {
     const int l = uni(lo[j]), u = uni(hi[j]), kk = k + dk[j];
     in[j] = en[j] && sj[j] >= 0 && l != kAbsent && l <= kk && kk <= u;
-    off[j] = c.ws[in[j] ? uni(kb[j]) + kk : ws_data(c)];
+    off[j] = ld_off(c, in[j] ? uni(kb[j]) + kk : ws_data(c));
   }
   long long m = LLONG_MIN;
 #pragma unroll
@@ -1361,7 +1584,7 @@ DI int base_case(Ctx<K>& c, int pb, int pe, int tb, int te, int cb, int ce, int*
   set_hdr(c, b, 0, 0, CM, cb == CM ? 0 : kAbsent, 0, ws_data(c));
   set_hdr(c, b, 0, 0, CI, cb == CI ? 0 : kAbsent, 0, ws_data(c));
   set_hdr(c, b, 0, 0, CD, cb == CD ? 0 : kAbsent, 0, ws_data(c));
-  c.ws[ws_data(c)] = 0;
+  st_off(c, ws_data(c), 0);
   team_sync<K>();
   int score = 0, ak;
   const int guard = score_guard(c, b.plen, b.tlen);
@@ -1859,7 +2082,7 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
 
 template <class K>
 __global__ __launch_bounds__(K::kNW > 1 ? K::kNW * kWave : kWave * kWavesPerBlock)
-__attribute__((amdgpu_waves_per_eu(4, 8)))
+__attribute__((amdgpu_waves_per_eu(K::kLds ? 4 : 3, 8)))
 void biwfa_persistent(KParams p) {
   extern __shared__ int32_t lds[];
   __shared__ int32_t tred[K::kNW > 1 ? 2 * K::kNW * kRedSlots : 1];
@@ -1870,6 +2093,7 @@ void biwfa_persistent(KParams p) {
   int32_t* ws = K::kLds ? lds + wv * p.L.total : p.gws + team * p.gws_stride;
   Ctx<K> c;
   c.ws = ws;
+  c.wd = ws;   // the layout puts the offsets first (L.data == 0)
   c.pv = ws + p.L.priv + wid * p.L.priv_ints;
   c.lane = lane;
   c.wid = wid;
@@ -1976,6 +2200,17 @@ void biwfa_persistent(KParams p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Explicit instantiations and the small kernels.  The build compiles this file
+// once per group (BW_PART = 0 .. kParts-1) so the groups build in parallel;
+// without BW_PART one translation unit holds everything.
+#if !defined(BW_PART)
+#define BW_GROUP(n) 1
+#else
+#define BW_GROUP(n) (BW_PART == (n))
+#endif
+
+#if BW_GROUP(0)
 // Sets every tier's {queue, count} counters to their initial values (a kernel
 // rather than a memcpy/memset so that the launch sequence can be captured in a graph).
 // Sets every tier's counters for a new launch sequence and advances the call
@@ -2049,12 +2284,27 @@ __global__ __launch_bounds__(256) void biwfa_compact(const uint32_t* runs, const
 }
 
 template __global__ void biwfa_persistent<Cfg<true, 1>>(KParams);
+#endif
+#if BW_GROUP(1)
 template __global__ void biwfa_persistent<Cfg<true, 1, kLdsDefaultWidth>>(KParams);
+#endif
+#if BW_GROUP(2)
 template __global__ void biwfa_persistent<Cfg<true, 1, kLdsDefaultWidth, true>>(KParams);
+#endif
+#if BW_GROUP(3)
 template __global__ void biwfa_persistent<Cfg<false, 1>>(KParams);
+#endif
+#if BW_GROUP(4)
 template __global__ void biwfa_persistent<Cfg<false, 2>>(KParams);
+#endif
+#if BW_GROUP(5)
 template __global__ void biwfa_persistent<Cfg<false, 4>>(KParams);
+#endif
+#if BW_GROUP(6)
 template __global__ void biwfa_persistent<Cfg<false, 8>>(KParams);
+#endif
+#if BW_GROUP(7)
 template __global__ void biwfa_persistent<Cfg<false, 16>>(KParams);
+#endif
 
 }  // namespace bw

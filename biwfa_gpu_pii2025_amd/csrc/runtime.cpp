@@ -65,8 +65,8 @@ int next_pow2(int v) {
   return p;
 }
 
-// Workspace layout of one aligner (int32 units): the shared ring / arena, then
-// one private region per wave of the team.  Wavefront limits live in
+// Workspace layout of one aligner (int32 units): the shared ring / arena,
+// then one private region per wave of the team.  Wavefront limits live in
 // registers; only the global-workspace kernels keep base-case headers in
 // memory (mem_base_hdr), because their base cases may exceed one register lane
 // per score.
@@ -74,10 +74,10 @@ bw::WsLayout make_layout(int scope, int wcap, int arena_min, int hcap, int rcap,
                          int seq_words, bool mem_base_hdr, int team_waves) {
   bw::WsLayout L{};
   int off = 0;
-  L.data = off;
+  L.data = off;   // the kernels index the offsets from the workspace base
   L.wcap = wcap;
-  const int ring_ints = 2 * scope * 3 * wcap;
-  L.arena_cap = std::max(ring_ints, arena_min);
+  const int ring_offsets = 2 * scope * 3 * wcap;
+  L.arena_cap = std::max(ring_offsets, arena_min);
   off += L.arena_cap;
   off = (off + 15) & ~15;
   L.priv = off;
@@ -466,9 +466,12 @@ static Tier make_global_tier(const bw_aligner_t* a, int gi, int wcap, bool last,
   Tier t;
   t.lds = false;
   t.nw = nw > 0 ? nw : tu.team_waves[gi] > 0 ? tu.team_waves[gi] : team_waves(a, wcap);
+  // base-case arena: every score's three components, each padded to whole
+  // groups of 4 offsets (up to 6 more) from a multiple-of-4 index (up to 3)
   const int width = std::min(2 * smax / q.e + 3, max_pt + 3);
-  const int64_t worst = (int64_t)3 * ((smax + g) / g) * width;
-  const int64_t arena = last ? worst : std::min<int64_t>(worst, (int64_t)3 * ((smax + g) / g) * std::min(wcap, width));
+  const int64_t worst = (int64_t)3 * ((smax + g) / g) * (width + 9);
+  const int64_t arena =
+      last ? worst : std::min<int64_t>(worst, (int64_t)3 * ((smax + g) / g) * (std::min(wcap, width) + 9));
   const int steps = std::min(max_pt, smax / std::max(1, std::min(q.x, q.e)) + 1);
   t.L = make_layout(scope, wcap, (int)std::min<int64_t>(arena, (int64_t)1 << 30), smax + 2,
                     2 * steps + 16, 64, 0, true, t.nw);
