@@ -266,6 +266,25 @@ DI void team_max8(Ctx<K>& c, int& v0, int& v1, int& v2, int& v3, int& v4, int& v
   v0 = rdl(r, 0); v1 = rdl(r, 1); v2 = rdl(r, 2); v3 = rdl(r, 3);
   v4 = rdl(r, 4); v5 = rdl(r, 5); v6 = rdl(r, 6); v7 = rdl(r, 7);
 }
+// In-place max over the team of sixteen wave-uniform values (one barrier), as
+// team_max8 with lanes 0..15 publishing.
+template <class K>
+DI void team_max16(Ctx<K>& c, int (&v)[16]) {
+  int32_t* b = team_buf(c);
+  const int l = c.lane;
+  int mine = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) mine |= v[j] & -(int)(l == j);
+  if (l < 16) b[c.wid * kRedSlots + l] = mine;
+  __syncthreads();
+  int r = INT_MIN;
+  if (l < 16) {
+#pragma unroll
+    for (int w = 0; w < K::kNW; ++w) r = imax(r, b[w * kRedSlots + l]);
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) v[j] = rdl(r, j);
+}
 template <class K>
 DI int team_max1(Ctx<K>& c, int v) {
   int32_t* b = team_buf(c);
@@ -662,14 +681,15 @@ DI int4 mask4(int4 v, const Wf& w, int k0) {
 // arena: the output allocation spans those groups).
 template <class K, bool kMax>
 DI void step_tiles(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, const Wf& mo, const Wf& ie,
-                   const Wf& de, bool has_i, bool has_d, int bm, int bi, int bd, int mask, StepAcc& acc) {
+                   const Wf& de, bool has_i, bool has_d, int bm, int bi, int bd, int mask, int wid, int nw,
+                   StepAcc& acc) {
   const bool ring = d.ring >= 0;
   const uint32_t tl = (uint32_t)d.tlen, pl = (uint32_t)d.plen;
   const int k_end = d.tlen - d.plen;
   const int ce = d.ce;
   const int t0 = lo & ~3;
   constexpr int kTile = 4 * kWave;
-  for (int kt = t0 + c.wid * kTile; kt <= hi; kt += K::kNW * kTile) {
+  for (int kt = t0 + wid * kTile; kt <= hi; kt += nw * kTile) {
     const int k0 = kt + 4 * c.lane;
     // every input read issued before any is used
     int4 vm = ld_off4(c, in4_index(mm, k0, ring));
@@ -762,6 +782,98 @@ DI void step_tiles(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, 
   }
 }
 
+// The whole step in one 64-diagonal chunk (the executing wave only):
+// branch-free input reads, trims straight from the ballots.  Writes this
+// wave's trims, maximum antidiagonal and end-diagonal offset.
+template <class K, bool kMax>
+DI void step_chunk(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, const Wf& mo, const Wf& ie,
+                   const Wf& de, bool has_i, bool has_d, int bm, int bi, int bd, int mask, int& fm, int& lm,
+                   int& fi, int& li, int& fd, int& ld, int& lmax, int& endu) {
+  const uint32_t tl = (uint32_t)d.tlen, pl = (uint32_t)d.plen;
+  const int k_end = d.tlen - d.plen;
+  const int ce = d.ce;
+  const int k = lo + c.lane;
+  const bool act = k <= hi;
+  int ins1, del1, mx;
+  if (d.ring >= 0) {
+    ins1 = imax(rd_ring(c, mo, k - 1), rd_ring(c, ie, k - 1)) + 1;
+    del1 = imax(rd_ring(c, mo, k + 1), rd_ring(c, de, k + 1));
+    mx = imax(del1, imax(rd_ring(c, mm, k) + 1, ins1));
+  } else {
+    ins1 = imax(rd_clamped(c, mo, k - 1), rd_clamped(c, ie, k - 1)) + 1;
+    del1 = imax(rd_clamped(c, mo, k + 1), rd_clamped(c, de, k + 1));
+    mx = imax(del1, imax(rd_clamped(c, mm, k) + 1, ins1));
+  }
+  const bool live = act && !((uint32_t)mx > tl || (uint32_t)(mx - k) > pl);
+  int ak = 0;
+  if (live) {                                   // extension along the diagonal
+    mx += lcp(c, d, mx - k, mx);
+    ak = 2 * mx - k;
+  }
+  mx = live ? mx : kOffsetNull;
+  if (act) {
+    st_off(c, bm + (k & mask), mx);
+    if (has_i) st_off(c, bi + (k & mask), ins1);
+    if (has_d) st_off(c, bd + (k & mask), del1);
+  }
+  const uint64_t b_m = __ballot(live);
+  const uint64_t b_i = __ballot(act && (uint32_t)ins1 <= tl && (uint32_t)(ins1 - k) <= pl);
+  const uint64_t b_d = __ballot(act && (uint32_t)del1 <= tl && (uint32_t)(del1 - k) <= pl);
+  if (b_m) { fm = lo + __builtin_ctzll(b_m); lm = lo + 63 - __builtin_clzll(b_m); }
+  if (b_i) { fi = lo + __builtin_ctzll(b_i); li = lo + 63 - __builtin_clzll(b_i); }
+  if (b_d) { fd = lo + __builtin_ctzll(b_d); ld = lo + 63 - __builtin_clzll(b_d); }
+  const int ev = (ce == CM) ? mx : (ce == CI) ? ins1 : del1;
+  if (k_end >= lo && k_end <= hi) endu = rdl(ev, k_end - lo);
+  if constexpr (kMax) lmax = wave_max_nonneg(ak);
+}
+
+// A global-tier step in tiles (wave `wid` of `nw` taking every nw-th tile),
+// reduced over the wave: trims, maximum antidiagonal and end-diagonal offset.
+template <class K, bool kMax>
+DI void step_tiles_reduced(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, const Wf& mo,
+                           const Wf& ie, const Wf& de, bool has_i, bool has_d, int bm, int bi, int bd, int mask,
+                           int wid, int nw, int& fm, int& lm, int& fi, int& li, int& fd, int& ld, int& lmax,
+                           int& endu) {
+  StepAcc acc{INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, 0, INT_MIN};
+  step_tiles<K, kMax>(c, d, lo, hi, mm, mo, ie, de, has_i, has_d, bm, bi, bd, mask, wid, nw, acc);
+  fm = wave_min_any(acc.fm); lm = wave_max_any(acc.lm);
+  fi = wave_min_any(acc.fi); li = wave_max_any(acc.li);
+  fd = wave_min_any(acc.fd); ld = wave_max_any(acc.ld);
+  // offset at k_end: lane ((k_end - (lo & ~3)) / 4) % 64 of the wave owning its tile
+  const int k_end = d.tlen - d.plen;
+  if (k_end >= lo && k_end <= hi) endu = rdl(acc.endv, ((k_end - (lo & ~3)) >> 2) & (kWave - 1));
+  if constexpr (kMax) lmax = wave_max_nonneg(acc.lmax);
+}
+
+// The end of a step once the team agrees on its values: trimmed limits into
+// the headers (wavefront_compute_trim_ends, wavefront_compute.c:579-613) and
+// the termination test (wavefront_termination_end2end, wavefront_termination.c:37-114).
+// Returns 1 when the end is reached; *max_ak gets the maximum antidiagonal.
+template <class K>
+DI int apply_step(Ctx<K>& c, Dir& d, int s, int slot, int lo, bool has_i, bool has_d, int bm, int bi, int bd,
+                  int fm, int lm, int fi, int li, int fd, int ld, int lmax, int endu, int* max_ak) {
+  // trimmed limits: hi = last in-bounds (else lo-1), lo = first in-bounds (else lo)
+  if (lm == INT_MIN) { fm = lo; lm = lo - 1; }
+  if (li == INT_MIN) { fi = lo; li = lo - 1; }
+  if (ld == INT_MIN) { fd = lo; ld = lo - 1; }
+  set_hdr(c, d, s, slot, CM, fm, lm, bm);
+  set_hdr(c, d, s, slot, CI, has_i ? fi : kAbsent, li, bi);
+  set_hdr(c, d, s, slot, CD, has_d ? fd : kAbsent, ld, bd);
+  // wavefront_termination_end2end: the end component holds k_end with offset >= tlen
+  const int k_end = d.tlen - d.plen;
+  const int ce = d.ce;
+  const bool ce_present = ce == CM || (ce == CI ? has_i : has_d);
+  const int ce_lo = ce == CM ? fm : ce == CI ? fi : fd;
+  const int ce_hi = ce == CM ? lm : ce == CI ? li : ld;
+  if (ce_present && ce_lo <= k_end && k_end <= ce_hi && endu >= d.tlen) {
+    d.status = kStEndReached;
+    d.status_score = s;
+    return 1;
+  }
+  *max_ak = lmax;
+  return 0;
+}
+
 // One WFA step at score s: wavefront_compute_affine (wavefront_compute_affine.c:229-260)
 // followed by wavefront_extend_end2end(_max) (wavefront_extend.c:86-211) and the
 // termination test (wavefront_termination.c:37-114), fused: the lane that
@@ -838,53 +950,14 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   if (width <= kWave) {
     // One chunk holds the whole wavefront (in the LDS tier always, for ring
     // widths up to 64; elsewhere the narrow steps of deep recursion levels and
-    // base cases): no loop, branch-free input reads, trims straight from the
-    // ballots.  In a team only wave 0 works; the others add neutral values to
-    // the team reduction.
-    if (K::kNW == 1 || c.wid == 0) {
-    const int k = lo + c.lane;
-    const bool act = k <= hi;
-    int ins1, del1, mx;
-    if (d.ring >= 0) {
-      ins1 = imax(rd_ring(c, mo, k - 1), rd_ring(c, ie, k - 1)) + 1;
-      del1 = imax(rd_ring(c, mo, k + 1), rd_ring(c, de, k + 1));
-      mx = imax(del1, imax(rd_ring(c, mm, k) + 1, ins1));
-    } else {
-      ins1 = imax(rd_clamped(c, mo, k - 1), rd_clamped(c, ie, k - 1)) + 1;
-      del1 = imax(rd_clamped(c, mo, k + 1), rd_clamped(c, de, k + 1));
-      mx = imax(del1, imax(rd_clamped(c, mm, k) + 1, ins1));
-    }
-    const bool live = act && !((uint32_t)mx > tl || (uint32_t)(mx - k) > pl);
-    int ak = 0;
-    if (live) {                                   // extension along the diagonal
-      mx += lcp(c, d, mx - k, mx);
-      ak = 2 * mx - k;
-    }
-    mx = live ? mx : kOffsetNull;
-    if (act) {
-      st_off(c, bm + (k & mask), mx);
-      if (has_i) st_off(c, bi + (k & mask), ins1);
-      if (has_d) st_off(c, bd + (k & mask), del1);
-    }
-    const uint64_t b_m = __ballot(live);
-    const uint64_t b_i = __ballot(act && (uint32_t)ins1 <= tl && (uint32_t)(ins1 - k) <= pl);
-    const uint64_t b_d = __ballot(act && (uint32_t)del1 <= tl && (uint32_t)(del1 - k) <= pl);
-    if (b_m) { fm = lo + __builtin_ctzll(b_m); lm = lo + 63 - __builtin_clzll(b_m); }
-    if (b_i) { fi = lo + __builtin_ctzll(b_i); li = lo + 63 - __builtin_clzll(b_i); }
-    if (b_d) { fd = lo + __builtin_ctzll(b_d); ld = lo + 63 - __builtin_clzll(b_d); }
-    const int ev = (ce == CM) ? mx : (ce == CI) ? ins1 : del1;
-    if (k_end >= lo && k_end <= hi) endu = rdl(ev, k_end - lo);
-    if constexpr (kMax) lmax = wave_max_nonneg(ak);
-    }
+    // base cases).  In a team only wave 0 works; the others add neutral values
+    // to the team reduction.
+    if (K::kNW == 1 || c.wid == 0)
+      step_chunk<K, kMax>(c, d, lo, hi, mm, mo, ie, de, has_i, has_d, bm, bi, bd, mask, fm, lm, fi, li, fd, ld,
+                          lmax, endu);
   } else if (!K::kLds && (d.ring < 0 || (hi | 3) - (lo & ~3) + 1 <= wcap(c))) {
-    StepAcc acc{INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, 0, INT_MIN};
-    step_tiles<K, kMax>(c, d, lo, hi, mm, mo, ie, de, has_i, has_d, bm, bi, bd, mask, acc);
-    fm = wave_min_any(acc.fm); lm = wave_max_any(acc.lm);
-    fi = wave_min_any(acc.fi); li = wave_max_any(acc.li);
-    fd = wave_min_any(acc.fd); ld = wave_max_any(acc.ld);
-    // offset at k_end: lane ((k_end - (lo & ~3)) / 4) % 64 of the wave owning its tile
-    if (k_end >= lo && k_end <= hi) endu = rdl(acc.endv, ((k_end - (lo & ~3)) >> 2) & (kWave - 1));
-    if constexpr (kMax) lmax = wave_max_nonneg(acc.lmax);
+    step_tiles_reduced<K, kMax>(c, d, lo, hi, mm, mo, ie, de, has_i, has_d, bm, bi, bd, mask, c.wid, K::kNW, fm, lm,
+                                fi, li, fd, ld, lmax, endu);
   } else {
   // A wave takes kG of its chunks at a time and issues all their loads before
   // any store: the stores of one chunk could alias the next chunk's inputs as
@@ -956,24 +1029,7 @@ DI int compute_extend(Ctx<K>& c, Dir& d, int s, int slot, int* max_ak) {
   } else {
     wave_sync<K::kLds>();
   }
-  // trimmed limits: hi = last in-bounds (else lo-1), lo = first in-bounds (else lo)
-  if (lm == INT_MIN) { fm = lo; lm = lo - 1; }
-  if (li == INT_MIN) { fi = lo; li = lo - 1; }
-  if (ld == INT_MIN) { fd = lo; ld = lo - 1; }
-  set_hdr(c, d, s, slot, CM, fm, lm, bm);
-  set_hdr(c, d, s, slot, CI, has_i ? fi : kAbsent, li, bi);
-  set_hdr(c, d, s, slot, CD, has_d ? fd : kAbsent, ld, bd);
-  // wavefront_termination_end2end: the end component holds k_end with offset >= tlen
-  const bool ce_present = ce == CM || (ce == CI ? has_i : has_d);
-  const int ce_lo = ce == CM ? fm : ce == CI ? fi : fd;
-  const int ce_hi = ce == CM ? lm : ce == CI ? li : ld;
-  if (ce_present && ce_lo <= k_end && k_end <= ce_hi && endu >= d.tlen) {
-    d.status = kStEndReached;
-    d.status_score = s;
-    return 1;
-  }
-  *max_ak = lmax;
-  return 0;
+  return apply_step(c, d, s, slot, lo, has_i, has_d, bm, bi, bd, fm, lm, fi, li, fd, ld, lmax, endu, max_ak);
 }
 
 // ---------------------------------------------------------------------------
@@ -1305,6 +1361,70 @@ DI void overlap(Ctx<K>& c, const Dir& a0, const Dir& a1, int s0, int slot0, int 
   }
 }
 
+// Phase 1 of the breakpoint search on a team: forward step s and reverse step s
+// read only their own direction's rings, so the two halves of the team compute
+// them at once (each half in tiles, or its first wave for a step of at most 64
+// diagonals) and one team reduction serves both; the headers of both are then
+// set.  Returns false, having changed nothing, when either step is null or its
+// ring slot is nearly full; the caller then steps one direction at a time.
+// *fq / *rq: the direction reached its end; *akf / *akr: maximum antidiagonals.
+template <class K>
+DI bool compute_extend_pair(Ctx<K>& c, Dir& f, Dir& r, int s, int qf, int qr, int& fq, int& rq, int& akf,
+                            int& akr) {
+  Wf fmm, fmo, fie, fde, rmm, rmo, rie, rde;
+  int flo = 0, fhi = 0, rlo = 0, rhi = 0;
+  bool fhas_i = false, fhas_d = false, rhas_i = false, rhas_d = false;
+  if (!gather_inputs(c, f, s, qf, fmm, fmo, fie, fde, flo, fhi, fhas_i, fhas_d)) return false;
+  if (!gather_inputs(c, r, s, qr, rmm, rmo, rie, rde, rlo, rhi, rhas_i, rhas_d)) return false;
+  const int w = wcap(c);
+  if ((fhi | 3) - (flo & ~3) + 1 > w || (rhi | 3) - (rlo & ~3) + 1 > w) return false;
+  f.null_steps = 0;
+  r.null_steps = 0;
+  const int mask = w - 1;
+  const int fbm = ring_base(c, f.ring, qf, CM), fbi = ring_base(c, f.ring, qf, CI), fbd = ring_base(c, f.ring, qf, CD);
+  const int rbm = ring_base(c, r.ring, qr, CM), rbi = ring_base(c, r.ring, qr, CI), rbd = ring_base(c, r.ring, qr, CD);
+  constexpr int kHalf = K::kNW / 2;
+  const bool rside = c.wid >= kHalf;
+  const int hw = rside ? c.wid - kHalf : c.wid;
+  int fm = INT_MAX, lm = INT_MIN, fi = INT_MAX, li = INT_MIN, fd = INT_MAX, ld = INT_MIN, lmax = 0, endu = INT_MIN;
+  if (!rside) {
+    if (fhi - flo + 1 <= kWave) {
+      if (hw == 0)
+        step_chunk<K, true>(c, f, flo, fhi, fmm, fmo, fie, fde, fhas_i, fhas_d, fbm, fbi, fbd, mask, fm, lm, fi, li,
+                            fd, ld, lmax, endu);
+    } else {
+      step_tiles_reduced<K, true>(c, f, flo, fhi, fmm, fmo, fie, fde, fhas_i, fhas_d, fbm, fbi, fbd, mask, hw, kHalf,
+                                  fm, lm, fi, li, fd, ld, lmax, endu);
+    }
+  } else {
+    if (rhi - rlo + 1 <= kWave) {
+      if (hw == 0)
+        step_chunk<K, true>(c, r, rlo, rhi, rmm, rmo, rie, rde, rhas_i, rhas_d, rbm, rbi, rbd, mask, fm, lm, fi, li,
+                            fd, ld, lmax, endu);
+    } else {
+      step_tiles_reduced<K, true>(c, r, rlo, rhi, rmm, rmo, rie, rde, rhas_i, rhas_d, rbm, rbi, rbd, mask, hw, kHalf,
+                                  fm, lm, fi, li, fd, ld, lmax, endu);
+    }
+  }
+  // slots 0..7 the forward step's values, 8..15 the reverse step's (INT_MIN is
+  // neutral for every one of them under max)
+  const int own[8] = {-fm, lm, -fi, li, -fd, ld, lmax, endu};
+  int v[16];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    v[j] = rside ? INT_MIN : own[j];
+    v[8 + j] = rside ? own[j] : INT_MIN;
+  }
+  team_max16(c, v);
+  akf = 0;
+  akr = 0;
+  fq = apply_step(c, f, s, qf, flo, fhas_i, fhas_d, fbm, fbi, fbd, -v[0], v[1], -v[2], v[3], -v[4], v[5], v[6], v[7],
+                  &akf);
+  rq = apply_step(c, r, s, qr, rlo, rhas_i, rhas_d, rbm, rbi, rbd, -v[8], v[9], -v[10], v[11], -v[12], v[13], v[14],
+                  v[15], &akr);
+  return true;
+}
+
 // wavefront_aligner_init_wf (wavefront_aligner.c:311-383) for a ring aligner.
 template <class K>
 DI void ring_init(Ctx<K>& c, Dir& d) {
@@ -1339,6 +1459,41 @@ DI int find_breakpoint(Ctx<K>& c, Dir& f, Dir& r, BP& bp) {
   for (;;) {
     if (f_max + r_max >= max_antidiagonal) break;
     if (sf + sr > c.guard) return kStGuard;
+    if constexpr (!K::kLds && K::kNW > 1) {
+      // Both next steps at once when they are not null by parity.  The reverse
+      // step then runs before WFA2's check between the two; when that check
+      // ends phase 1, the step is undone: it wrote the slot of score s - scope,
+      // a score that is not a multiple of g (scope = max(o+e, x) + 1 is 1 mod g),
+      // hence null and absent, and its headers and the direction's state are
+      // restored (its ring data is never read while absent).
+      const int s2 = sf + 1;
+      if (sf == sr && c.gmask != 0 && (s2 & c.gmask) == 0) {
+        const int qf2 = next_slot(qf, c.scope), qr2 = next_slot(qr, c.scope);
+        const int j = r.ring * c.scope + qr2;
+        const int h0 = rdl(c.rlo[CM], j), h1 = rdl(c.rlo[CI], j), h2 = rdl(c.rlo[CD], j);
+        const int u0 = rdl(c.rhi[CM], j), u1 = rdl(c.rhi[CI], j), u2 = rdl(c.rhi[CD], j);
+        const Dir rsave = r;
+        int fq = 0, rq = 0, akf = 0, akr = 0;
+        if (compute_extend_pair(c, f, r, s2, qf2, qr2, fq, rq, akf, akr)) {
+          sf = s2; qf = qf2;
+          if (f_max < akf) f_max = akf;
+          last_forward = true;
+          if (fq) return f.status;
+          if (f_max + r_max >= max_antidiagonal) {   // WFA2 stops here, before the reverse step
+            r = rsave;
+            lane_set(c.rlo, CM, c.lane, j, h0); lane_set(c.rlo, CI, c.lane, j, h1); lane_set(c.rlo, CD, c.lane, j, h2);
+            lane_set(c.rhi, CM, c.lane, j, u0); lane_set(c.rhi, CI, c.lane, j, u1); lane_set(c.rhi, CD, c.lane, j, u2);
+            break;
+          }
+          sr = s2; qr = qr2;
+          if (r_max < akr) r_max = akr;
+          last_forward = false;
+          if (rq) return r.status;
+          if (sr + sf >= c.max_steps) return kStMaxSteps;
+          continue;
+        }
+      }
+    }
     ++sf; qf = next_slot(qf, c.scope);
     const int fq = compute_extend(c, f, sf, qf, &ak);
     if (c.spill) return kStCapacity;
