@@ -226,6 +226,7 @@ struct bw_aligner {
   int64_t ready_ticks = kReadyTicksDefault; // bound of a shared-task ready-flag wait (100 MHz ticks)
   int withhold_ready = 0;                   // failure-path test hook (bw_test_set_ready_guard)
   int32_t* scratch = nullptr;               // this device's no-allocation scratch (bw_align_device_noalloc)
+  int64_t ws_budget = 0;                    // bytes for the global tiers' workspace slabs
   int graph = 0;                   // replay a captured launch sequence (bw_set_graph)
   bool last_graph = false;         // the last sequence was a graph replay (no per-tier events)
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
@@ -334,6 +335,8 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
   a->pen = pen;
   a->dev = attr.device;
   a->cu_count = prop.multiProcessorCount;
+  // workspace slabs of the global tiers: at most a third of the device memory, 64 GiB
+  a->ws_budget = std::min<int64_t>((int64_t)64 << 30, (int64_t)(prop.totalGlobalMem / 3));
   a->lds_per_block_max = prop.sharedMemPerBlock;
   a->schedule = attr.schedule;
   if (hipSetDevice(a->dev) != hipSuccess || hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -526,12 +529,16 @@ static int check_lds_layout(const bw_aligner_t* a, const Tier& t0) {
 static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b) {
   std::vector<Tier> tiers;
   tiers.push_back(make_lds_tier(a));
-  // global tiers: ring width 256, 1024, ... up to the batch maximum
+  // Global tiers: single-wave aligners with rings 256 and 1024 wide, then one
+  // team tier as wide as the batch needs.  Team tiers of graded widths would
+  // only save workspace memory (every team's slab holds its tier's ring), at
+  // the price of a tail at every tier boundary and re-runs of the pairs that
+  // outgrow a tier; one slab of the widest ring per team fits the budget.
   const int wmax = next_pow2(std::max(b->max_pt + 8, 256));
-  const int64_t budget = (int64_t)24 << 30;
+  const int64_t budget = a->ws_budget;
   const int max_seq_words = (std::max(b->max_p, b->max_t) + 31) / 32 + 3;
   for (int w = 256;; w *= 4) {
-    const bool last = w >= wmax || (int)tiers.size() + 1 >= kMaxTiers;
+    const bool last = w >= wmax || w > 1024 || (int)tiers.size() + 1 >= kMaxTiers;
     const int wcap = last ? wmax : w;
     tiers.push_back(make_global_tier(a, (int)tiers.size() - 1, wcap, last, b->max_pt, max_seq_words, budget));
     if (last || (int)tiers.size() >= kMaxTiers) break;
