@@ -3,24 +3,27 @@
 
 A step is one pass of the hot path over one batch: every pair of the workload is
 aligned (score + CIGAR) by the persistent kernels, inputs already resident in HBM
-and outputs left there.  Default workload: the short-read batch (1,000,000 pairs
-x 150 bp, 2% edits, x=4 o=6 e=2, length-sorted schedule).
+and outputs left there.  Default workload: the uneven mixed batch (200,000 pairs,
+lengths log-uniform 100-50,000 bp, 1-15 % edits, x=4 o=6 e=2) under the learned
+schedule, with the plain length-sorted schedule timed on the same batch in the
+same run (fewer steps, stated in the line) and both outputs compared.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload C2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload C4]
 
 Prints one JSON line.  Besides the contract fields it carries
-  roofline      -- HBM roofline of the dominant kernel (algorithmic bytes / kernel time),
-  issue_roofline -- instruction issue rate of that kernel (the bound it runs against:
-                   one SALU and one VALU wave instruction per CU per clock; the
-                   busier of the two is reported), from the kernel's per-pair SQ
-                   counts in profiles/ and the live kernel time,
-  int_roofline  -- the algorithmic integer work (cells and compares) against the
-                   int32 VALU peak,
-  cpu_baseline  -- WFA2-lib BiWFA (built from the reference sources) with OpenMP
-                   on this host's cores, on a bounded sample of the same workload,
-  parity        -- GPU vs CPU reference on that sample (scores and CIGARs).
+  schedules     -- plain vs learned: ms/step of each, learned_vs_plain, identical_results,
+                   cost-model training and prediction figures,
+  roofline      -- algorithmic int32 work (11 ops per wavefront cell + 4 per 64-bit word
+                   compared, counted by the CPU restatement) per step over the total
+                   kernel time, against the int32 VALU peak; names the dominant kernel,
+  hbm           -- compulsory HBM bytes and the dominant kernel's measured HBM traffic
+                   (committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes) per kernel second,
+  cpu_baseline  -- WFA2-lib BiWFA (built from the reference sources) with OpenMP on all
+                   of this host's cores, on a deterministic subsample of the same workload,
+  parity        -- GPU vs CPU reference on that subsample (scores and CIGARs).
 """
 import argparse
+import glob
 import json
 import os
 import sys
@@ -32,136 +35,137 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0                  # MI355X HBM3E
-# instruction issue: one scalar (SALU) and one vector (VALU) wave instruction per CU
-# per clock, 256 CUs at 2.4 GHz
-ISSUE_PEAK_G = 256 * 2.4
-# int32 VALU peak: 256 CUs x 4 SIMD x 16 lanes per clock at 2.4 GHz
-INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+CLOCK_GHZ = 2.4                        # MI355X peak engine clock
+# int32 VALU peak: 256 CUs x 64 lanes per clock
+INT32_PEAK_TOPS = 256 * 64 * CLOCK_GHZ * 1e9 / 1e12
+# LDS: 256 CUs x 128 bytes per clock
+LDS_PEAK_GBS = 256 * 128 * CLOCK_GHZ
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(p_len, t_len, nruns):
-    """Compulsory HBM bytes: packed pattern+text words, per-pair offsets/lengths/order
-    (24+4 B) read; score, wf_score, status, run count (16 B) and 4 B per CIGAR run written."""
+def compulsory_bytes(p_len, t_len, nruns):
+    """Compulsory HBM bytes of a step: packed pattern+text words, per-pair offsets,
+    lengths and order (24+4 B) read; score, wf_score, status, run count (16 B) and
+    4 B per CIGAR run written."""
     words = (p_len.astype(np.int64) + 31) // 32 + (t_len.astype(np.int64) + 31) // 32
     return int(np.sum(8 * words) + len(p_len) * (24 + 4 + 16) + 4 * int(np.sum(nruns)))
 
 
-def traffic_bytes(workload, n):
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's readings), when they
-    were taken on this same workload; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "c2_traffic.json")
+def profile_json(kind, workload, n):
+    """A committed profiles/*_<kind>.json summary taken on this same workload, or None."""
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_%s.json" % kind))):
+        try:
+            with open(path) as f:
+                t = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if t.get("workload") == workload and int(t.get("pairs", -1)) == n:
+            t["_path"] = os.path.relpath(path, ROOT)
+            return t
+    return None
+
+
+def host_cpu():
+    """Logical cores, the cores this process may run on, physical cores and the model."""
+    model, cores = None, set()
+    phys = None
     try:
-        with open(path) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    if t.get("workload") != workload or int(t.get("pairs", -1)) != n:
-        return None
-    return int(t["traffic_bytes"])
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    cores.add((phys, v))
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"logical": os.cpu_count(), "usable": usable, "physical": len(cores) or None, "model": model}
 
 
-def sq_counts(workload, n):
-    """Per-pair SALU / VALU wave-instruction counts of the dominant kernel from the
-    committed rocprofv3 SQ counter pass, when it was taken on this same workload."""
-    path = os.path.join(ROOT, "profiles", "c2_sq.json")
-    try:
-        with open(path) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    if t.get("workload") != workload or int(t.get("pairs", -1)) != n:
-        return None
-    return t
-
-
-def _per_pair_mismatches(off_a, runs_a, off_b, runs_b, m):
-    """Number of pairs among the first m whose run lists differ."""
-    if np.array_equal(off_a[:m + 1], off_b[:m + 1]) and \
-            np.array_equal(runs_a[:off_a[m]], runs_b[:off_b[m]]):
+def _mismatched_pairs(off_a, runs_a, off_b, runs_b):
+    """Number of pairs whose run lists differ (same pair order on both sides)."""
+    if np.array_equal(off_a, off_b) and np.array_equal(runs_a[:off_a[-1]], runs_b[:off_b[-1]]):
         return 0
-    bad = 0
-    for i in range(m):
-        if not np.array_equal(runs_a[off_a[i]:off_a[i + 1]], runs_b[off_b[i]:off_b[i + 1]]):
-            bad += 1
-    return bad
+    return sum(1 for i in range(len(off_a) - 1)
+               if not np.array_equal(runs_a[off_a[i]:off_a[i + 1]], runs_b[off_b[i]:off_b[i + 1]]))
 
 
-def cpu_baseline(ds, res, min_core_s=12.0, threads=None):
-    """Times the reference BiWFA on a bounded sample of the workload (repeating the
-    sample until about min_core_s core-seconds of CPU work are timed) and checks the
-    GPU output against it; also runs the oracle restatement for the work counters."""
+def gather_runs(res, idx):
+    """CIGAR run offsets and runs of the pairs idx of a device result, compacted."""
+    lens = (res.cigar_offset[idx + 1] - res.cigar_offset[idx]).astype(np.int64)
+    off = np.zeros(len(idx) + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    runs = np.concatenate([res.cigar_runs[res.cigar_offset[i]:res.cigar_offset[i + 1]] for i in idx]) \
+        if len(idx) else np.zeros(0, np.uint32)
+    return off, runs
+
+
+def cpu_baseline(ds, res, threads, target_wall_s=8.0, every=None):
+    """Times the reference BiWFA on all `threads` host cores over a deterministic
+    subsample of the workload (every k-th pair, k sized for about target_wall_s of
+    wall time; the sample is repeated when one pass is shorter) and checks the GPU
+    output on the subsample against it."""
     from oracle.oracle import Oracle, Wfa2Ref
-    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count())
     kind = "reference" if Wfa2Ref.available() else "port"
     eng = Wfa2Ref() if kind == "reference" else Oracle()
     n = len(ds)
 
-    def view(m):
-        return (ds.seq, ds.p_off[:m], ds.p_len[:m], ds.t_off[:m], ds.t_len[:m])
-    # calibrate on a growing slice (8, 16, 32, ... pairs until it takes about half a
-    # second), then size the sample to ~min_core_s core-seconds
-    probe = min(n, 8)
-    while True:
+    def view(idx):
+        return (ds.seq, ds.p_off[idx], ds.p_len[idx], ds.t_off[idx], ds.t_len[idx])
+    if every is None:
+        # calibrate on every 1000th pair (at least 64 pairs), then size the stride
+        probe = np.arange(0, n, max(1, n // max(64, n // 1000)), dtype=np.int64)
         r = eng.align(view(probe), threads=threads)
-        if r.elapsed >= 0.5 or probe >= n or probe >= 20000:
-            break
-        probe = min(n, 2 * probe)
-    per_pair_core_s = max(r.elapsed * threads / probe, 1e-8)
-    m = int(min(n, max(probe, min_core_s / per_pair_core_s)))
-    if m > probe:
-        r = eng.align(view(m), threads=threads)
+        per_pair_wall = max(r.elapsed / len(probe), 1e-9)
+        m = int(min(n, max(64, target_wall_s / per_pair_wall)))
+        every = max(1, n // m)
+    idx = np.arange(0, n, every, dtype=np.int64)
+    eng.align(view(idx[:max(1, len(idx) // 100)]), threads=threads)     # 1 % warm-up
+    r = eng.align(view(idx), threads=threads)
     wall, reps = r.elapsed, 1
-    while wall * threads < min_core_s:
-        wall += eng.align(view(m), threads=threads).elapsed
+    while wall < target_wall_s * 0.5 and reps < 64:
+        wall += eng.align(view(idx), threads=threads).elapsed
         reps += 1
-    cpu_pairs_s = m * reps / wall
-    # parity against the GPU output on the sample
-    s_bad = int(np.sum(res.score[:m] != r.alg_score[:m]))
+    cpu_pairs_s = len(idx) * reps / wall
+    s_bad = int(np.sum(res.score[idx] != r.alg_score))
     ref_off, ref_runs = r.compact()
-    c_bad = _per_pair_mismatches(res.cigar_offset, res.cigar_runs, ref_off, ref_runs, m)
-    # algorithmic work counters (cells, 64-bit words) from the restatement
-    cnt_m = min(m, 100000, max(8, int(4.0 / per_pair_core_s)))   # a few core-seconds of counting
-    oc = Oracle().align(view(cnt_m), threads=threads)
-    o_off, o_runs = oc.compact()
-    o_bad = _per_pair_mismatches(o_off, o_runs, ref_off, ref_runs, cnt_m) + \
-        int(np.sum(oc.alg_score != r.alg_score[:cnt_m]))
+    g_off, g_runs = gather_runs(res, idx)
+    c_bad = _mismatched_pairs(g_off, g_runs, ref_off, ref_runs)
     return {
-        "value": round(cpu_pairs_s, 1), "unit": "pairs/s", "cores": threads, "kind": kind,
-        "sample": "first %d pairs of the same workload, %d pass(es), %.1f s wall on %d threads (%s)" % (
-            m, reps, wall, threads, "WFA2-lib v2.3 BiWFA built from the reference sources, "
-            "OpenMP dynamic,16, one aligner per thread" if kind == "reference"
+        "value": round(cpu_pairs_s, 2), "unit": "pairs/s", "cores": threads, "kind": kind,
+        "sample": "%d pairs (every %d-th pair of the workload), %d pass(es), %.1f s wall on %d threads; %s" % (
+            len(idx), every, reps, wall, threads, "WFA2-lib v2.3 BiWFA built from the reference sources, "
+            "ultralow memory, OpenMP dynamic,16, one aligner per thread" if kind == "reference"
             else "C restatement, OpenMP dynamic,16"),
-    }, {
-        "checked": m, "score_mismatches": s_bad, "cigar_mismatches": c_bad,
-        "oracle_vs_reference_mismatches": o_bad,
-    }, {
-        "cells_per_pair": float(np.mean(oc.cells)), "words_per_pair": float(np.mean(oc.words)),
-        "pairs_counted": cnt_m,
-    }
+    }, {"checked": int(len(idx)), "every": int(every), "score_mismatches": s_bad, "cigar_mismatches": c_bad}, idx
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="C2")
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="C4")
     ap.add_argument("--pairs", type=int, default=None, help="first N pairs of the workload")
-    ap.add_argument("--schedule", default="length", choices=["length", "input"])
+    ap.add_argument("--schedule", default=None, choices=["learned", "length", "input"],
+                    help="default: learned (with plain length-sorted timed beside it) for C4, else length")
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="timed steps of the plain length-sorted schedule beside the learned one")
+    ap.add_argument("--plain-warmup", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="wall seconds of the CPU baseline")
     ap.add_argument("--autotune", type=int, default=0, metavar="N",
                     help="tune launch shapes on an N-pair sample of the workload before timing")
     ap.add_argument("--tuning", default=None, help="JSON launch tuning (bw_tuning_t fields)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the captured launch sequence (bw_set_graph); per-tier times read 0")
-    ap.add_argument("--compare-schedules", action="store_true",
-                    help="train the cost model, then time the plain length-sorted and the learned "
-                         "schedule on the same batch (value = learned)")
     ap.add_argument("--train-pairs", type=int, default=None,
                     help="training pairs for the cost model (default: the full 50,000-pair set)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -182,16 +186,19 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    t_start_all = time.time()
     import biwfa_gpu_pii2025_amd as bw
     cfg = bw.CONFIGS[args.workload]
     n = args.pairs or cfg.num_pairs
+    schedule = args.schedule or ("learned" if args.workload == "C4" else "length")
+    compare = schedule == "learned"
     t0 = time.time()
     ds = bw.generate_dataset(cfg, num_pairs=n)
     packed = ds.pack()
     log("generated %d pairs of %s in %.1f s" % (n, args.workload, time.time() - t0))
 
-    sched = bw.SCHEDULE_LENGTH_SORTED if args.schedule == "length" else bw.SCHEDULE_INPUT_ORDER
-    aligner = bw.WavefrontAligner(penalties=bw.configs.PENALTIES, schedule=sched)
+    base_sched = bw.SCHEDULE_INPUT_ORDER if schedule == "input" else bw.SCHEDULE_LENGTH_SORTED
+    aligner = bw.WavefrontAligner(penalties=bw.configs.PENALTIES, schedule=base_sched)
     if args.tuning:
         aligner.tuning = json.loads(args.tuning)
     if args.graph:
@@ -204,31 +211,33 @@ def main():
         log("autotuned on %d pairs in %.1f s: %s" % (len(sample), time.time() - t0, json.dumps(tuned)))
     db = aligner.upload(packed)
 
-    def timed(label):
-        for w in range(args.warmup):
+    def timed(label, steps, warmup):
+        for w in range(warmup):
             db.align()
             log("%swarmup %d done (%.2f ms)" % (label, w, db.timing()[0]))
-        km, ks, kt, ktiers = [], [], [], []
-        t_start = time.perf_counter()
-        for k in range(args.steps):
+        km, kt, ktiers = [], [], []
+        t_a = time.perf_counter()
+        for k in range(steps):
             db.align()
-            tot, main_ms, spill_ms = db.timing()
-            kt.append(tot); km.append(main_ms); ks.append(spill_ms)
+            tot, main_ms, _ = db.timing()
+            kt.append(tot); km.append(main_ms)
             ktiers.append(aligner.tier_timing())
-        t_end = time.perf_counter()
-        ms = (t_end - t_start) * 1000.0 / args.steps
+            if steps > 2:
+                log("%sstep %d: %.2f ms" % (label, k, tot))
+        t_b = time.perf_counter()
+        ms = (t_b - t_a) * 1000.0 / steps
         tiers = [round(float(v), 4) for v in np.mean(np.array(ktiers), axis=0)]
-        log("%stimed %d steps: %.3f ms/step (kernels %.3f ms: main %.3f + spill %.3f; tiers %s; "
-            "tier pairs %s)" % (label, args.steps, ms, np.mean(kt), np.mean(km), np.mean(ks), tiers,
-                                 aligner.tier_pairs()))
-        return ms, km, ks, kt, tiers
+        log("%stimed %d steps: %.3f ms/step (kernels %.3f ms; tiers %s; tier pairs %s)" % (
+            label, steps, ms, np.mean(kt), tiers, aligner.tier_pairs()))
+        return ms, float(np.mean(kt)), tiers
 
     sched_info = None
-    if args.compare_schedules:
+    plain_res = None
+    if compare:
         from biwfa_gpu_pii2025_amd import costmodel as cm
-        plain_ms = timed("plain: ")[0]
+        plain_ms, plain_kms, plain_tiers = timed("plain: ", args.plain_steps, args.plain_warmup)
         plain_res = db.download()
-        plain_tuning = aligner.tuning
+        plain_pairs = aligner.tier_pairs()
         db.free()
         t0 = time.time()
         model, tinfo = cm.train_cost_model(num_pairs=args.train_pairs, epochs=30, seed=0)
@@ -236,10 +245,8 @@ def main():
         log("cost model trained on %d aligner-labelled pairs in %.1f s (mse %.5f -> %.5f)" % (
             tinfo["pairs"], train_s, tinfo["initial_mse"], tinfo["final_mse"]))
         if sample is not None:
-            t0 = time.time()
             spred = model.predict_score(cm.batch_features(sample)).astype(np.float32)
-            tuned = aligner.autotune(sample, repeats=2, predicted_score=spred, progress=log)
-            log("learned schedule autotuned in %.1f s: %s" % (time.time() - t0, json.dumps(tuned)))
+            aligner.autotune(sample, repeats=2, predicted_score=spred, progress=log)
         db = aligner.upload(packed)
         t0 = time.time()
         pred = cm.learned_schedule(db, model, packed)
@@ -248,33 +255,41 @@ def main():
         err = np.abs(np.log2(1 + pred) - np.log2(1 + np.maximum(true_score, 0)))
         rank_p = np.argsort(np.argsort(pred))
         rank_t = np.argsort(np.argsort(true_score))
-        spearman = float(np.corrcoef(rank_p, rank_t)[0, 1])
-        sched_info = {"plain_ms_per_step": round(plain_ms, 4), "plain_pairs_s": round(n / plain_ms * 1e3, 1),
-                      "plain_tuning": plain_tuning,
-                      "train_pairs": tinfo["pairs"], "train_s": round(train_s, 2),
-                      "train_mse": [round(tinfo["initial_mse"], 6), round(tinfo["final_mse"], 6)],
-                      "predict_and_schedule_s": round(sched_s, 3),
-                      "score_prediction": {"median_abs_log2_err": round(float(np.median(err)), 4),
-                                           "spearman": round(spearman, 4)}}
+        sched_info = {
+            "plain": {"schedule": "length-sorted", "steps": args.plain_steps, "warmup": args.plain_warmup,
+                      "ms_per_step": round(plain_ms, 3), "pairs_s": round(n / plain_ms * 1e3, 2),
+                      "kernel_ms": round(plain_kms, 3), "tier_ms": plain_tiers, "tier_pairs": plain_pairs},
+            "cost_model": {"train_pairs": tinfo["pairs"], "train_s": round(train_s, 2),
+                           "train_mse": [round(tinfo["initial_mse"], 6), round(tinfo["final_mse"], 6)],
+                           "predict_and_schedule_s": round(sched_s, 3),
+                           "median_abs_log2_err": round(float(np.median(err)), 4),
+                           "spearman": round(float(np.corrcoef(rank_p, rank_t)[0, 1]), 4)}}
         model.close()
-    ms_per_step, kern_main, kern_spill, kern_total, tier_ms = timed("learned: " if args.compare_schedules else "")
+    ms_per_step, kern_ms, tier_ms = timed("learned: " if compare else "", args.steps, args.warmup)
+    tier_pairs = aligner.tier_pairs()
+    shapes = aligner.tier_shapes()
     res = db.download()
     if sched_info is not None:
-        same = (np.array_equal(res.score, plain_res.score) and np.array_equal(res.cigar_offset, plain_res.cigar_offset)
+        same = (np.array_equal(res.score, plain_res.score) and np.array_equal(res.wf_score, plain_res.wf_score)
+                and np.array_equal(res.cigar_offset, plain_res.cigar_offset)
                 and np.array_equal(res.cigar_runs, plain_res.cigar_runs))
-        sched_info["learned_pairs_s"] = round(n / ms_per_step * 1e3, 1)
-        sched_info["learned_vs_plain"] = round(sched_info["plain_ms_per_step"] / ms_per_step, 3)
+        sched_info["learned"] = {"ms_per_step": round(ms_per_step, 3), "pairs_s": round(n / ms_per_step * 1e3, 2),
+                                 "tier_pairs": tier_pairs}
+        sched_info["learned_vs_plain"] = round(sched_info["plain"]["ms_per_step"] / ms_per_step, 4)
         sched_info["identical_results"] = bool(same)
     bad_status = int(np.sum(res.status != 0))
     nruns = np.diff(res.cigar_offset)
-
     value = n / (ms_per_step / 1000.0)
-    main_s = float(np.mean(kern_main)) / 1000.0
-    alg_bytes = algorithmic_bytes(packed.p_len, packed.t_len, nruns)
-    achieved_gbs = alg_bytes / main_s / 1e9
+    kern_s = kern_ms / 1000.0
+
+    # dominant kernel: the tier with the largest share of the step's kernel time
+    dom = int(np.argmax(tier_ms)) if tier_ms else 0
+    dom_shape = shapes[dom] if dom < len(shapes) else {}
+    dom_share = float(tier_ms[dom]) / max(1e-9, float(sum(tier_ms))) if tier_ms else None
+
     out = {
         "metric": "pairs/s (score+CIGAR, bit-exact) on 1 MI355X; learned vs plain schedule; vs CPU",
-        "value": round(value, 1),
+        "value": round(value, 2),
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -286,58 +301,84 @@ def main():
         "dtype": "int32",
         "data": "synthetic (seeded generator, seed %d)" % cfg.seed,
         "config": {
-            "workload": "%s: %s; %d pairs; gap-affine x=4 o=6 e=2; score+CIGAR; %s schedule" % (
+            "workload": "%s: %s; %d pairs; gap-affine x=4 o=6 e=2; score+CIGAR; %s" % (
                 cfg.name, cfg.description, n,
-                "learned (cost model)" if args.compare_schedules else
-                "length-sorted" if args.schedule == "length" else "input-order"),
+                "learned (cost-model) schedule; plain length-sorted timed beside it on %d step(s)" % args.plain_steps
+                if compare else "length-sorted schedule" if schedule == "length" else "input-order schedule"),
             "pairs": n,
+            "schedule": schedule,
             "graph_replay": bool(args.graph),
             "pattern_length": [int(packed.p_len.min()), int(packed.p_len.max())],
             "penalties": list(bw.configs.PENALTIES),
         },
         "gcups": round(float(np.sum(packed.p_len.astype(np.float64) * packed.t_len)) / (ms_per_step / 1000.0) / 1e9, 3),
-        "kernel_ms": {"total": round(float(np.mean(kern_total)), 4), "main": round(float(np.mean(kern_main)), 4),
-                      "spill": round(float(np.mean(kern_spill)), 4), "tiers": tier_ms},
+        "kernel_ms": {"total": round(kern_ms, 4), "tiers": tier_ms, "tier_pairs": tier_pairs},
+        "tiers": shapes,
         "tuning": aligner.tuning,
         "schedules": sched_info,
         "status_errors": bad_status,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved_gbs, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-            "traffic": traffic_bytes(args.workload, n),
-            "kernel": "biwfa_persistent<Cfg<true,1,32,true>> (LDS tier, one wave per pair, ring width 32, default penalties)",
-            "algorithmic_bytes_per_launch": alg_bytes,
-        },
     }
-    sq = sq_counts(args.workload, n)
-    if sq is not None:
-        salu_g = sq["salu_per_pair"] * n / main_s / 1e9
-        valu_g = sq["valu_per_pair"] * n / main_s / 1e9
-        top = max(salu_g, valu_g)
-        out["issue_roofline"] = {
-            "bound": "valu-issue" if valu_g >= salu_g else "salu-issue", "achieved": round(top, 1),
-            "peak": ISSUE_PEAK_G, "unit": "G instr/s", "frac": round(top / ISSUE_PEAK_G, 4),
-            "salu_frac": round(salu_g / ISSUE_PEAK_G, 4), "valu_frac": round(valu_g / ISSUE_PEAK_G, 4),
-            "salu_per_pair": round(sq["salu_per_pair"], 1), "valu_per_pair": round(sq["valu_per_pair"], 1),
-            "counts_from": "profiles/biwfa_persistent_lds_C2_1M_sq_counters.md",
+
+    # algorithmic int32 work of the step (counted by the CPU restatement)
+    counts = profile_json("work_counts", args.workload, n)
+    s2_all = float(np.sum(res.score.astype(np.float64) ** 2))
+    if counts is not None:
+        ops = s2_all * counts["ops_per_score_sq"]
+        cells = s2_all * counts["sum_cells"] / counts["sum_score_sq"]
+        words = s2_all * counts["sum_words"] / counts["sum_score_sq"]
+        achieved = ops / kern_s / 1e12
+        out["roofline"] = {
+            "bound": "valu-int32", "achieved": round(achieved, 4), "peak": round(INT32_PEAK_TOPS, 2),
+            "unit": "Tops/s", "frac": round(achieved / INT32_PEAK_TOPS, 5),
+            "traffic": None,
+            "kernel": dom_shape.get("kernel"), "kernel_share_of_step": round(dom_share, 4) if dom_share else None,
+            "time": "total kernel time of the step (all tiers), HIP events",
+            "ops_per_step": ops, "cells_per_step": cells, "words_per_step": words,
+            "ops_model": counts["ops_model"],
+            "counts_from": "%s (CPU restatement on %d sampled pairs; whole batch by sum(score^2) x ops/score^2, "
+                           "half-sample spread %.2g)" % (counts["_path"], counts["sample_pairs"],
+                                                          counts["half_sample_ratio_spread"]),
+            "peak_clock_ghz": CLOCK_GHZ,
         }
+        out["cells_per_s"] = round(cells / kern_s, 1)
+        lds_bytes = 32.0 * cells + 16.0 * words
+        out["lds_model"] = {"achieved": round(lds_bytes / kern_s / 1e9, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(lds_bytes / kern_s / 1e9 / LDS_PEAK_GBS, 5),
+                            "model": "32 B per cell + 16 B per word"}
+    comp = compulsory_bytes(packed.p_len, packed.t_len, nruns)
+    hbm = {"compulsory_bytes": comp, "compulsory_gbs": round(comp / kern_s / 1e9, 3), "peak": HBM_PEAK_GBS}
+    traffic = profile_json("traffic", args.workload, n)
+    if traffic is not None and traffic.get("kernel") == dom_shape.get("kernel"):
+        tb = int(traffic["traffic_bytes"])
+        dom_s = float(tier_ms[dom]) / 1000.0
+        hbm.update({"kernel": traffic["kernel"], "traffic_bytes": tb,
+                    "measured_gbs": round(tb / dom_s / 1e9, 1),
+                    "measured_frac": round(tb / dom_s / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic_per_cell": round(tb / out["roofline"]["cells_per_step"], 2) if "roofline" in out else None,
+                    "from": traffic["_path"]})
+        if "roofline" in out:
+            out["roofline"]["traffic"] = tb
+    out["hbm"] = hbm
+
     if not args.no_cpu:
         t0 = time.time()
-        cpu, parity, counters = cpu_baseline(ds, res)
-        log("cpu baseline %.0f pairs/s on %d threads (%.1f s)" % (cpu["value"], cpu["cores"], time.time() - t0))
-        out["cpu_baseline"] = cpu
-        out["gpu_vs_cpu"] = round(value / cpu["value"], 2)
+        cpu = host_cpu()
+        threads = cpu["usable"]
+        every = 100 if (args.workload == "C4" and n == cfg.num_pairs) else None
+        base, parity, idx = cpu_baseline(ds, res, threads, target_wall_s=args.cpu_seconds, every=every)
+        base.update({"logical_cores": cpu["logical"], "physical_cores": cpu["physical"], "cpu_model": cpu["model"]})
+        # the subsample's mean work against the whole batch's (work ~ score^2)
+        s2 = res.score.astype(np.float64) ** 2
+        ratio = float(np.mean(s2[idx]) / max(1e-9, np.mean(s2)))
+        base["sample_work_vs_batch"] = round(ratio, 4)
+        base["value_batch_equivalent"] = round(base["value"] * ratio, 2)
+        log("cpu baseline %.1f pairs/s on %d threads (%.1f s); sample work / batch work %.3f" % (
+            base["value"], threads, time.time() - t0, ratio))
+        out["cpu_baseline"] = base
+        out["gpu_vs_cpu"] = round(value / base["value"], 2)
+        out["gpu_vs_cpu_batch_equivalent"] = round(value / base["value_batch_equivalent"], 2)
         out["parity"] = parity
-        ops_per_pair = 11.0 * counters["cells_per_pair"] + 4.0 * counters["words_per_pair"]
-        achieved_tops = ops_per_pair * n / main_s / 1e12
-        out["int_roofline"] = {
-            "bound": "valu-int32", "achieved": round(achieved_tops, 4), "peak": round(INT32_PEAK_TOPS, 2),
-            "unit": "Tops/s", "frac": round(achieved_tops / INT32_PEAK_TOPS, 5),
-            "ops_per_pair": round(ops_per_pair, 1), "counters": counters,
-        }
+    log("bench done in %.1f s" % (time.time() - t_start_all))
     print(json.dumps(out), flush=True)
 
 

@@ -79,6 +79,13 @@ class _DevResult(ctypes.Structure):
                 ("runs", ctypes.c_void_p), ("run_offset", ctypes.c_void_p)]
 
 
+class _TierShape(ctypes.Structure):
+    """bw_tier_shape_t: launch shape of one tier of the last device-batch alignment."""
+    _fields_ = [("lds", ctypes.c_int), ("waves_per_aligner", ctypes.c_int), ("ring_width", ctypes.c_int),
+                ("blocks", ctypes.c_int), ("waves_per_block", ctypes.c_int), ("aligners", ctypes.c_int64),
+                ("kernel", ctypes.c_char * 96)]
+
+
 MAX_TIERS = 8
 _PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
 
@@ -136,6 +143,7 @@ def lib():
         L.bw_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Tuning)]
         L.bw_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Tuning)]
         L.bw_last_tier_timing.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.bw_last_tier_shape.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_TierShape)]
         L.bw_last_tier_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.bw_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch), ctypes.c_void_p, ctypes.c_int,
                                   ctypes.POINTER(_Tuning)]
@@ -522,6 +530,20 @@ class WavefrontAligner:
         cnt = np.zeros(MAX_TIERS, np.int32)
         n = self._L.bw_last_tier_pairs(self._h, _ptr(cnt), MAX_TIERS)
         return cnt[:max(0, n)].tolist()
+
+    def tier_shapes(self):
+        """Launch shape of each tier of the last device-batch alignment: dicts of lds,
+        waves_per_aligner, ring_width, blocks, waves_per_block, aligners, kernel."""
+        out = []
+        for t in range(MAX_TIERS):
+            sh = _TierShape()
+            if self._L.bw_last_tier_shape(self._h, t, ctypes.byref(sh)) != 0:
+                break
+            out.append({"lds": bool(sh.lds), "waves_per_aligner": sh.waves_per_aligner,
+                        "ring_width": sh.ring_width, "blocks": sh.blocks,
+                        "waves_per_block": sh.waves_per_block, "aligners": int(sh.aligners),
+                        "kernel": sh.kernel.decode()})
+        return out
 
 
 class DeviceBatch:

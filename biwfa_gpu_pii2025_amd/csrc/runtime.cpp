@@ -217,6 +217,7 @@ struct bw_aligner {
   int last_ntiers = 0;
   float tier_ms[kMaxTiers] = {};
   int32_t tier_pairs[kMaxTiers] = {};   // pairs each tier took on (initial + spilled in)
+  bw_tier_shape_t tier_shape[kMaxTiers] = {};   // launch shapes of the last sequence
   bw_tuning_t tuning{};
   bw_progress_fn progress = nullptr;
   void* progress_user = nullptr;
@@ -795,6 +796,41 @@ static int launch_sequence(bw_aligner_t* a, bw_dbatch_t* b, bool events) {
   return 0;
 }
 
+// Names of the kernel instantiations as profilers print them.
+static const char* kernel_name(const Tier& T, bool defpen) {
+  if (T.lds) {
+    if (T.L.wcap == bw::kLdsDefaultWidth && defpen) return "bw::biwfa_persistent<bw::Cfg<true, 1, 32, true>>";
+    if (T.L.wcap == bw::kLdsDefaultWidth) return "bw::biwfa_persistent<bw::Cfg<true, 1, 32, false>>";
+    return "bw::biwfa_persistent<bw::Cfg<true, 1, 0, false>>";
+  }
+  switch (T.nw) {
+    case 1: return "bw::biwfa_persistent<bw::Cfg<false, 1, 0, false>>";
+    case 2: return "bw::biwfa_persistent<bw::Cfg<false, 2, 0, false>>";
+    case 4: return "bw::biwfa_persistent<bw::Cfg<false, 4, 0, false>>";
+    case 8: return "bw::biwfa_persistent<bw::Cfg<false, 8, 0, false>>";
+    default: return "bw::biwfa_persistent<bw::Cfg<false, 16, 0, false>>";
+  }
+}
+
+static void record_shape(bw_aligner_t* a, int t, const Tier& T) {
+  const bool defpen = default_penalties(a->pen);
+  bw_tier_shape_t& sh = a->tier_shape[t];
+  sh = bw_tier_shape_t{};
+  {
+    sh.lds = T.lds ? 1 : 0;
+    sh.waves_per_aligner = T.lds ? 1 : T.nw;
+    sh.ring_width = T.L.wcap;
+    sh.blocks = T.blocks;
+    sh.waves_per_block = T.waves_per_block;
+    sh.aligners = T.lds ? (int64_t)T.blocks * T.waves_per_block : T.aligners;
+    std::snprintf(sh.kernel, sizeof(sh.kernel), "%s", kernel_name(T, defpen));
+  }
+}
+
+static void record_shapes(bw_aligner_t* a, const bw_dbatch_t* b) {
+  for (int t = 0; t < (int)b->tiers.size() && t < kMaxTiers; ++t) record_shape(a, t, b->tiers[t]);
+}
+
 int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
   if (!a || !b) return BW_STATUS_INVALID_ARG;
   if (const int st = check_owner(a, b)) return st;
@@ -820,6 +856,7 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
     }
     HIP_TRY(hipEventRecord(a->ev[0], s));
     HIP_TRY(hipEventRecord(a->ev[1], s));
+    record_shapes(a, b);
     HIP_TRY(hipGraphLaunch(b->gexec, s));
     HIP_TRY(hipEventRecord(a->ev[2], s));
     HIP_TRY(hipEventRecord(a->ev[3], s));
@@ -827,6 +864,7 @@ int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
     a->last_graph = true;
     return 0;
   }
+  record_shapes(a, b);
   HIP_TRY(hipEventRecord(a->ev[0], s));
   const int st = launch_sequence(a, b, true);
   if (st != 0) return st;
@@ -979,6 +1017,12 @@ int bw_last_tier_pairs(const bw_aligner_t* a, int32_t* pairs, int capacity) {
   if (!a) return BW_STATUS_INVALID_ARG;
   for (int t = 0; t < a->last_ntiers && t < capacity; ++t) pairs[t] = a->tier_pairs[t];
   return a->last_ntiers;
+}
+
+int bw_last_tier_shape(const bw_aligner_t* a, int tier, bw_tier_shape_t* shape) {
+  if (!a || !shape || tier < 0 || tier >= a->last_ntiers) return BW_STATUS_INVALID_ARG;
+  *shape = a->tier_shape[tier];
+  return 0;
 }
 
 // Runs the sample under tuning `t` (upload, one warm-up, `repeats` timed runs)
@@ -1336,6 +1380,7 @@ int bw_align_device_noalloc(bw_aligner_t* a, const bw_batch_t* in, const bw_devi
   const Tier* tiers[2] = {&t0, &t1};
   for (int t = 0; t < 2; ++t) {
     const Tier& T = *tiers[t];
+    record_shape(a, t, T);
     bw::KParams q = p;
     q.list = t == 0 ? list0 : list1;
     q.qctl = counters + bw::kQctlInts * t;

@@ -233,6 +233,20 @@ int bw_last_tier_timing(const bw_aligner_t* aligner, float* ms, int capacity);
  * Valid after bw_synchronize.  Returns the number of tiers. */
 int bw_last_tier_pairs(const bw_aligner_t* aligner, int32_t* pairs, int capacity);
 
+/* Launch shape of one tier of the last bw_dbatch_align (no counterpart in the
+ * reference; for reports that name the kernel a tier ran).  `kernel` is the
+ * instantiation's name as profilers print it. */
+typedef struct {
+  int lds;                 /* 1: the LDS tier (workspace in LDS), 0: a global tier */
+  int waves_per_aligner;   /* team size: waves that align one pair together */
+  int ring_width;          /* diagonals per ring slot */
+  int blocks;              /* workgroups launched */
+  int waves_per_block;
+  int64_t aligners;        /* aligners resident at once */
+  char kernel[96];
+} bw_tier_shape_t;
+int bw_last_tier_shape(const bw_aligner_t* aligner, int tier, bw_tier_shape_t* shape);
+
 /* Times the sample batch under candidate tunings and keeps the fastest: team
  * waves per global tier (each tier timed on its own), then the LDS tier's queue
  * chunk and ring width (whole batch).  The sample runs under the aligner's

@@ -1,0 +1,88 @@
+"""Algorithmic work of a workload, counted by the C restatement (oracle/biwfa_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the counts feed bench.py's integer roofline; nothing on
+the product path reads them.
+
+Per pair the restatement counts
+  C = sum over every compute step (forward, reverse, every recursion level, every
+      base case) of hi - lo + 1 (the limits of wavefront_compute_limits_input,
+      wavefront_compute.c:219-260),
+  W = sum over every extended non-null diagonal of floor(run / 32) + 1 (64-bit
+      words of 2-bit packed bases compared),
+  R = CIGAR runs,
+and the algorithmic int32 work is O = 11 C + 4 W (11 two-input integer ops per
+cell in wavefront_compute_affine.c:65-85: I = max + add, D = max, M = add + max +
+max, bounds = sub + 2 compares + 2 selects; 4 per word: xor, zero test, ctz, add).
+
+The full mixed-length workload takes hours on the CPU, so the counts come from a
+deterministic subsample (every k-th pair).  C is proportional to s^2 (s the
+alignment score) within a few percent, so bench.py estimates the whole batch's O
+as sum_all(s^2) * sum_sample(O) / sum_sample(s^2) (a ratio estimator; its spread
+over sample halves is reported with the counts).
+
+    python -m oracle.work_counts C4 --every 100 --out profiles/c4_200k_work_counts.json
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def count(workload, every, threads=0, num_pairs=None):
+    import biwfa_gpu_pii2025_amd as bw
+    from oracle.oracle import Oracle
+    cfg = bw.CONFIGS[workload]
+    n = num_pairs or cfg.num_pairs
+    ds = bw.generate_dataset(cfg, num_pairs=n)
+    idx = np.arange(0, n, every, dtype=np.int64)
+    view = (ds.seq, ds.p_off[idx], ds.p_len[idx], ds.t_off[idx], ds.t_len[idx])
+    t0 = time.time()
+    r = Oracle().align(view, threads=threads)
+    el = time.time() - t0
+    c = np.asarray(r.cells, np.float64)
+    w = np.asarray(r.words, np.float64)
+    s2 = np.asarray(r.alg_score, np.float64) ** 2
+    o = 11.0 * c + 4.0 * w
+    # spread of the ratio estimator: the two halves of the sample against each other
+    half = np.arange(len(idx)) % 2 == 0
+    ra, rb = o[half].sum() / s2[half].sum(), o[~half].sum() / s2[~half].sum()
+    return {
+        "workload": workload, "pairs": int(n), "seed": int(cfg.seed), "every": int(every),
+        "sample_pairs": int(len(idx)), "sample": "pairs i with i %% %d == 0" % every,
+        "sum_cells": float(c.sum()), "sum_words": float(w.sum()), "sum_runs": int(np.sum(r.nruns)),
+        "sum_ops": float(o.sum()), "sum_score_sq": float(s2.sum()),
+        "ops_per_score_sq": float(o.sum() / s2.sum()),
+        "half_sample_ratio_spread": float(abs(ra - rb) / (0.5 * (ra + rb))),
+        "cells_per_score_sq_cv": float(np.std(c / np.maximum(s2, 1)) / np.mean(c / np.maximum(s2, 1))),
+        "status_nonzero": int(np.sum(r.status != 0)),
+        "oracle_seconds": round(el, 1), "threads": threads or os.cpu_count(),
+        "ops_model": "O = 11*C + 4*W int32 ops (C cells, W 64-bit words compared)",
+        "command": "python -m oracle.work_counts %s --every %d" % (workload, every),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload")
+    ap.add_argument("--every", type=int, default=100)
+    ap.add_argument("--pairs", type=int, default=None)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    res = count(a.workload, a.every, a.threads, a.pairs)
+    text = json.dumps(res, indent=1)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(text + "\n")
+    print(text)
+
+
+if __name__ == "__main__":
+    main()
