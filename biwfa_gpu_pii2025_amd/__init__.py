@@ -157,6 +157,17 @@ def lib():
                                           ctypes.c_void_p, ctypes.c_void_p]
         L.bw_pack_2bit.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.bw_seqs_read.restype = ctypes.c_void_p
+        L.bw_seqs_read.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.bw_seqs_count.restype = ctypes.c_int64
+        L.bw_seqs_count.argtypes = [ctypes.c_void_p]
+        L.bw_seqs_get.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p),
+                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_void_p),
+                                  ctypes.POINTER(ctypes.c_int)]
+        L.bw_seqs_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch)]
+        L.bw_seqs_free.argtypes = [ctypes.c_void_p]
+        L.bw_alg_write.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_int]
         L.bw_align_pair.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_int,
@@ -258,6 +269,83 @@ def pack_pairs(pairs):
     buf = np.frombuffer(b"".join(chunks) + b"\0", np.uint8)
     return pack_ascii(buf, np.array(p_off, np.int64), np.array(p_len, np.int32),
                       np.array(t_off, np.int64), np.array(t_len, np.int32))
+
+
+# ---------------------------------------------------------------------------
+# sequence files and .alg output
+
+SEQ_FORMAT_AUTO, SEQ_FORMAT_WFA2, SEQ_FORMAT_REPO = 0, 1, 2
+
+
+class SeqFile:
+    """The pairs of a sequence file, read by the library (bw_seqs_read): WFA2's .seq
+    format (a pattern line and a text line per pair, each behind a one-character
+    marker) or the repo driver's input (header "N P T", then N line pairs)."""
+
+    def __init__(self, path, fmt=SEQ_FORMAT_AUTO):
+        self._L = lib()
+        self._h = self._L.bw_seqs_read(os.fsencode(path), int(fmt))
+        if not self._h:
+            raise BiwfaError("bw_seqs_read: " + last_error())
+
+    def __len__(self):
+        return int(self._L.bw_seqs_count(self._h))
+
+    def pair(self, i):
+        p, t = ctypes.c_void_p(), ctypes.c_void_p()
+        pl, tl = ctypes.c_int(), ctypes.c_int()
+        if self._L.bw_seqs_get(self._h, int(i), ctypes.byref(p), ctypes.byref(pl), ctypes.byref(t),
+                               ctypes.byref(tl)) != 0:
+            raise IndexError(i)
+        return ctypes.string_at(p, pl.value).decode(), ctypes.string_at(t, tl.value).decode()
+
+    def pairs(self):
+        return [self.pair(i) for i in range(len(self))]
+
+    def packed(self):
+        """The pairs 2-bit packed by the library (copied into a PackedPairs)."""
+        b = _Batch()
+        st = self._L.bw_seqs_batch(self._h, ctypes.byref(b))
+        if st != 0:
+            raise BiwfaError("bw_seqs_batch: " + last_error())
+        n, nw = b.num_pairs, b.num_words
+
+        def arr(ptr, ct, count, dt):
+            if count == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), shape=(count,)).astype(dt)
+        return PackedPairs(arr(b.words, ctypes.c_uint64, nw, np.uint64),
+                           arr(b.pattern_word_offset, ctypes.c_int64, n, np.int64),
+                           arr(b.pattern_length, ctypes.c_int32, n, np.int32),
+                           arr(b.text_word_offset, ctypes.c_int64, n, np.int64),
+                           arr(b.text_length, ctypes.c_int32, n, np.int32))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bw_seqs_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_alg(path, result, score_only=False):
+    """Writes align_benchmark's lite .alg file (bw_alg_write) from a BatchResult."""
+    n = len(result.score)
+    st = lib().bw_alg_write(os.fsencode(path), n, _ptr(result.score),
+                            None if score_only else _ptr(result.cigar_offset),
+                            None if score_only else _ptr(result.cigar_runs), int(bool(score_only)))
+    if st != 0:
+        raise BiwfaError("bw_alg_write: " + last_error())
 
 
 # ---------------------------------------------------------------------------

@@ -63,7 +63,7 @@ def build(verbose=False, jobs=None):
     for src in ("runtime.cpp", "costmodel.hip"):
         obj = os.path.join(OBJ, src.split(".")[0] + ".o")
         steps.append(([hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
-    for src in ("dataset.cpp",):
+    for src in ("dataset.cpp", "seqio.cpp"):
         obj = os.path.join(OBJ, src.split(".")[0] + ".o")
         steps.append((["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
     jobs = jobs or max(1, min(len(steps), os.cpu_count() or 1, 16))

@@ -132,6 +132,11 @@ int internal_penalties(const bw_affine_penalties_t& p, Penalties* out) {
 
 }  // namespace
 
+// The other host sources (sequence files) report through the same text.
+namespace bw_host {
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace bw_host
+
 // One launch of the persistent kernel over a list of pairs with fixed capacities.
 // Tier 0 keeps its workspace in LDS; later tiers use per-wave global slabs whose
 // ring width grows geometrically.  A pair that outgrows its tier (ring width,
@@ -199,6 +204,7 @@ struct bw_dbatch {
 };
 
 constexpr int kMaxTiers = 8;
+static void drop_one(bw_aligner_t* a);
 // Longest wait for a claimed shared task's ready flag: 60 s of the 100 MHz clock.
 constexpr int64_t kReadyTicksDefault = 60ll * 100000000ll;
 // Pairs with p + t below this run depth first on one aligner of a global tier.
@@ -228,6 +234,7 @@ struct bw_aligner {
   int withhold_ready = 0;                   // failure-path test hook (bw_test_set_ready_guard)
   int32_t* scratch = nullptr;               // this device's no-allocation scratch (bw_align_device_noalloc)
   int64_t ws_budget = 0;                    // bytes for the global tiers' workspace slabs
+  bw_dbatch_t* one = nullptr;               // single-pair calls: reused device batch (bw_align)
   int graph = 0;                   // replay a captured launch sequence (bw_set_graph)
   bool last_graph = false;         // the last sequence was a graph replay (no per-tier events)
   float last_total_ms = 0.f, last_main_ms = 0.f, last_spill_ms = 0.f;
@@ -366,6 +373,7 @@ void bw_aligner_delete(bw_aligner_t* a) {
   if (!a) return;
   (void)hipSetDevice(a->dev);
   if (a->stream) (void)hipStreamSynchronize(a->stream);
+  bw_dbatch_free(a->one);
   dfree(a->d_counters);
   for (auto& e : a->ev)
     if (e) (void)hipEventDestroy(e);
@@ -527,7 +535,7 @@ static int check_lds_layout(const bw_aligner_t* a, const Tier& t0) {
   return 0;
 }
 
-static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b) {
+static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b, int64_t budget) {
   std::vector<Tier> tiers;
   tiers.push_back(make_lds_tier(a));
   // Global tiers: single-wave aligners with rings 256 and 1024 wide, then one
@@ -536,7 +544,6 @@ static std::vector<Tier> plan_tiers(const bw_aligner_t* a, const bw_dbatch_t* b)
   // the price of a tail at every tier boundary and re-runs of the pairs that
   // outgrow a tier; one slab of the widest ring per team fits the budget.
   const int wmax = next_pow2(std::max(b->max_pt + 8, 256));
-  const int64_t budget = a->ws_budget;
   const int max_seq_words = (std::max(b->max_p, b->max_t) + 31) / 32 + 3;
   for (int w = 256;; w *= 4) {
     const bool last = w >= wmax || w > 1024 || (int)tiers.size() + 1 >= kMaxTiers;
@@ -560,7 +567,7 @@ static int initial_tier(const bw_dbatch_t* b, int pair, double width_guess) {
 
 extern "C" {
 
-static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out) {
+static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out, int64_t budget = 0) {
   *out = nullptr;
   if (!a || !in || in->num_pairs < 0 || (in->num_pairs > 0 && (!in->words || !in->pattern_length ||
       !in->text_length || !in->pattern_word_offset || !in->text_word_offset))) {
@@ -596,7 +603,7 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out)
   }
   b->runs_cap = runs_off[n];
   b->owner = a->id;
-  b->tiers = plan_tiers(a, b);
+  b->tiers = plan_tiers(a, b, budget > 0 ? std::min(budget, a->ws_budget) : a->ws_budget);
   if (const int st = check_lds_layout(a, b->tiers[0])) {
     bw_dbatch_free(b);
     return st;
@@ -998,6 +1005,7 @@ int bw_set_tuning(bw_aligner_t* a, const bw_tuning_t* t) {
     }
   }
   a->tuning = *t;
+  drop_one(a);
   return 0;
 }
 
@@ -1072,6 +1080,7 @@ static int min_team_for(int wcap) {
 int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, int repeats,
                 bw_tuning_t* best) {
   if (!a || !sample) return BW_STATUS_INVALID_ARG;
+  drop_one(a);   // the single-pair batch was planned for the tuning in force
   const int sched0 = a->schedule;
   if (repeats < 1) repeats = 1;
   const bw_tuning_t start = a->tuning;
@@ -1136,46 +1145,102 @@ int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, in
 
 // ---------------- single pair (WFA2-style) ----------------
 
-static int align_one(bw_aligner_t* a, const std::vector<uint64_t>& words, int plen, int tlen) {
-  const int pw = (plen + 31) / 32, tw = (tlen + 31) / 32;
+// Workspace bytes of the reused single-pair batch: enough aligners for one
+// pair's shared recursion, far fewer than a full batch's.
+constexpr int64_t kOneBudget = (int64_t)512 << 20;
+
+// The reused device batch of the single-pair calls, sized for pairs up to
+// cap_p x cap_t (one zeroed pair of that shape is uploaded to plan it).
+static int reserve_one(bw_aligner_t* a, int cap_p, int cap_t) {
+  bw_dbatch_free(a->one);
+  a->one = nullptr;
+  const int pw = (cap_p + 31) / 32, tw = (cap_t + 31) / 32;
+  std::vector<uint64_t> words((size_t)1 + pw + tw + 2, 0ull);
   const int64_t pwoff = 1, twoff = 1 + pw;
   bw_batch_t bt;
   bt.num_pairs = 1;
   bt.num_words = (int64_t)words.size();
   bt.words = words.data();
   bt.pattern_word_offset = &pwoff;
-  bt.pattern_length = &plen;
+  bt.pattern_length = &cap_p;
   bt.text_word_offset = &twoff;
-  bt.text_length = &tlen;
-  (void)tw;
-  int32_t score = 0, wf = 0, status = 0;
-  int64_t off[2] = {0, 0};
-  a->runs.assign((size_t)plen + tlen + 1, 0u);
-  bw_result_t r{};
-  r.score = &score;
-  r.wf_score = &wf;
-  r.status = &status;
-  r.cigar_offset = off;
-  r.cigar_runs = a->runs.data();
-  r.cigar_capacity = (int64_t)a->runs.size();
-  const int st = bw_align_batch(a, &bt, &r);
-  if (st != 0) {
+  bt.text_length = &cap_t;
+  return upload_impl(a, &bt, &a->one, kOneBudget);
+}
+
+static void drop_one(bw_aligner_t* a) {
+  bw_dbatch_free(a->one);
+  a->one = nullptr;
+}
+
+// One pair through the reused device batch: its tiers and workspace are
+// planned once for the largest pair seen so far (the capacity grows by
+// doubling), and each call only copies the pair in, aligns and copies the
+// result out (one reused aligner per thread, as align_benchmark.c:521-523 does
+// with wavefront_aligner_new).
+static int align_one(bw_aligner_t* a, const std::vector<uint64_t>& words, int plen, int tlen) {
+  auto failed = [&](int st) {
     a->status = st;
     a->runs.clear();
     a->ops.clear();
     return st;
+  };
+  if (hipSetDevice(a->dev) != hipSuccess) return failed(BW_STATUS_HIP_ERROR);
+  bw_dbatch_t* b = a->one;
+  if (!b || plen > b->max_p || tlen > b->max_t) {
+    int cp = std::max(plen, 1024), ct = std::max(tlen, 1024);
+    if (b) {
+      cp = std::max(cp, plen > b->max_p ? 2 * b->max_p : b->max_p);
+      ct = std::max(ct, tlen > b->max_t ? 2 * b->max_t : b->max_t);
+    }
+    if (const int st = reserve_one(a, cp, ct)) return failed(st);
+    b = a->one;
   }
-  a->runs.resize((size_t)off[1]);
-  a->status = status;
-  a->score = score;
-  a->wf_score = wf;
+  const int pw = (plen + 31) / 32;
+  const int64_t pwoff = 1, twoff = 1 + pw;
+  hipStream_t s = a->stream;
+  if (hipMemcpyAsync(b->d_words, words.data(), 8 * words.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(b->d_pwoff, &pwoff, 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(b->d_twoff, &twoff, 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(b->d_plen, &plen, 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(b->d_tlen, &tlen, 4, hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_error("copy of the pair to the device failed");
+    return failed(BW_STATUS_HIP_ERROR);
+  }
+  b->h_plen[0] = plen;
+  b->h_tlen[0] = tlen;
+  const int sched = a->schedule == BW_SCHEDULE_LEARNED ? BW_SCHEDULE_LENGTH_SORTED : a->schedule;
+  int st = bw_set_schedule(a, b, sched, nullptr);     // (synchronizes the stream: the copies are done)
+  if (st == 0) st = bw_dbatch_align(a, b);
+  if (st == 0) st = bw_synchronize(a);
+  if (st != 0) return failed(st);
+  int32_t out[4] = {0, 0, 0, 0};
+  if (hipMemcpyAsync(&out[0], b->d_score, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&out[1], b->d_wf, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&out[2], b->d_status, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&out[3], b->d_nruns, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("copy of the result from the device failed");
+    return failed(BW_STATUS_HIP_ERROR);
+  }
+  // a single pair's runs start at the batch's first run slot (runs_off[0] = 0)
+  const int nr = std::max(0, std::min<int>(out[3], (int)b->runs_cap));
+  a->runs.assign((size_t)nr, 0u);
+  if (nr > 0 && (hipMemcpyAsync(a->runs.data(), b->d_runs, 4 * (size_t)nr, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                 hipStreamSynchronize(s) != hipSuccess)) {
+    set_error("copy of the CIGAR from the device failed");
+    return failed(BW_STATUS_HIP_ERROR);
+  }
+  a->status = out[2];
+  a->score = out[0];
+  a->wf_score = out[1];
   a->ops.clear();
   for (uint32_t run : a->runs) {
     const int op = run & 15u;
     const char ch = op == BW_OP_MATCH ? 'M' : op == BW_OP_MISMATCH ? 'X' : op == BW_OP_INSERTION ? 'I' : 'D';
     a->ops.insert(a->ops.end(), run >> 4, ch);
   }
-  return status;
+  return a->status;
 }
 
 static int pack_ascii(const char* seq, int len, uint64_t* w) {
@@ -1239,10 +1304,22 @@ int bw_get_sam_cigar(bw_aligner_t* a, int show_mismatches, const uint32_t** buf,
     *len = (int)a->runs.size();
     return 0;
   }
-  // fold X into M (SAM op 0) and merge (cigar_compute_CIGAR, cigar.c:181-229)
+  // fold X into M (SAM op 0) and merge, as cigar_compute_CIGAR does
+  // (cigar.c:181-229): the fold starts at the second operation, so a leading
+  // mismatch stays one X run of length 1
   a->sam.clear();
+  bool first = true;
   for (uint32_t run : a->runs) {
     uint32_t op = run & 15u, n = run >> 4;
+    if (n == 0) continue;
+    if (first && op == BW_OP_MISMATCH) {
+      a->sam.push_back((1u << 4) | BW_OP_MISMATCH);
+      if (--n == 0) {
+        first = false;
+        continue;
+      }
+    }
+    first = false;
     if (op == BW_OP_MATCH || op == BW_OP_MISMATCH) op = 0;
     if (!a->sam.empty() && (a->sam.back() & 15u) == op) a->sam.back() += n << 4;
     else a->sam.push_back((n << 4) | op);

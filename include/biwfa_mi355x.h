@@ -116,8 +116,10 @@ int bw_get_status(const bw_aligner_t* aligner);
  * like cigar->operations[begin_offset,end_offset).  Valid until the next call. */
 int bw_get_cigar_ops(const bw_aligner_t* aligner, const char** ops, int* length);
 
-/* SAM-packed CIGAR, cigar_get_CIGAR (alignment/cigar.c:230-240).  show_mismatches=1
- * gives the run encoding above; 0 folds X into M (op 0).  Valid until the next call. */
+/* SAM-packed CIGAR, cigar_get_CIGAR (alignment/cigar.c:181-240).  show_mismatches=1
+ * gives the run encoding above; 0 folds X into M (op 0) from the second operation
+ * on, as cigar_compute_CIGAR does (a leading mismatch stays a 1X run).  Valid until
+ * the next call. */
 int bw_get_sam_cigar(bw_aligner_t* aligner, int show_mismatches,
                      const uint32_t** cigar_buffer, int* cigar_length);
 
@@ -362,6 +364,35 @@ int bw_dataset_generate(const bw_dataset_params_t* params, const int64_t* patter
 /* Packs ASCII sequences into 2-bit words at word offsets (see bw_batch_t). */
 int bw_pack_2bit(int64_t num_seqs, const char* ascii, const int64_t* offset,
                  const int32_t* length, const int64_t* word_offset, uint64_t* words);
+
+/* ---------------- sequence files and .alg output ---------------- */
+
+#define BW_SEQ_FORMAT_AUTO 0   /* WFA2 when the file starts with '>' or '<', else the repo's */
+#define BW_SEQ_FORMAT_WFA2 1   /* .seq: pattern line, text line, each behind a one-char marker */
+#define BW_SEQ_FORMAT_REPO 2   /* header "N P T", then N pattern / text line pairs            */
+
+typedef struct bw_seqs bw_seqs_t;
+/* Reads every pair of a sequence file.  Replaces align_benchmark's input loop
+ * (align_benchmark_read_input, tools/align_benchmark/align_benchmark.c:313-338;
+ * the files generate_dataset writes, generate_dataset.c:387-393) and the repo
+ * driver's reader (src/biWFA.cu:776-835).  NULL on error (bw_last_error). */
+bw_seqs_t* bw_seqs_read(const char* path, int format);
+int64_t bw_seqs_count(const bw_seqs_t* seqs);
+/* Pair i's ASCII sequences (owned by seqs). */
+int bw_seqs_get(const bw_seqs_t* seqs, int64_t i, const char** pattern, int* pattern_length,
+                const char** text, int* text_length);
+/* The pairs as a 2-bit packed batch whose arrays belong to seqs (valid until
+ * bw_seqs_free or the next bw_seqs_batch); BW_STATUS_BAD_SEQUENCE for a byte
+ * outside ACGT. */
+int bw_seqs_batch(bw_seqs_t* seqs, bw_batch_t* batch);
+void bw_seqs_free(bw_seqs_t* seqs);
+/* Writes align_benchmark's lite .alg file, one "score<TAB>CIGAR" line per pair
+ * (benchmark_print_output_lite / benchmark_print_output,
+ * tools/align_benchmark/benchmark/benchmark_utils.c:102-117, 143-179): "-" for an
+ * empty CIGAR, and -1 as the score of a full alignment whose CIGAR is empty.
+ * With score_only, cigar_offset and cigar_runs are not read. */
+int bw_alg_write(const char* path, int64_t num_pairs, const int32_t* score, const int64_t* cigar_offset,
+                 const uint32_t* cigar_runs, int score_only);
 
 #ifdef __cplusplus
 }

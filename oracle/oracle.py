@@ -155,6 +155,16 @@ class Wfa2Ref(_Engine):
     def available(variant="v3"):
         return os.path.exists(os.path.join(REF_DIR, Wfa2Ref.VARIANTS[variant]))
 
+    def align(self, pairs, penalties=(0, 4, 6, 2), score_only=False, threads=0, max_steps=0,
+              show_mismatches=True):
+        """As _Engine.align; show_mismatches=False returns the runs of
+        cigar_get_CIGAR(cigar, false): X folded into M (op 0)."""
+        self.lib.wfa2ref_set_show_mismatches(ctypes.c_int(int(bool(show_mismatches))))
+        try:
+            return super().align(pairs, penalties, score_only, threads, max_steps)
+        finally:
+            self.lib.wfa2ref_set_show_mismatches(ctypes.c_int(1))
+
     def _call(self, n, buf, p_off, p_len, t_off, t_len, m, x, o, e, score_only, max_steps,
               threads, res, el):
         self.lib.wfa2ref_align_batch(

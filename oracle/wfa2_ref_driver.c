@@ -23,8 +23,10 @@
  *                -1 when the CIGAR is empty.
  *   raw_score  : aligner->cigar->score as left by wavefront_align.
  *   status     : wavefront_align's return code (wfa.h:46-55).
- *   runs       : cigar_get_CIGAR(cigar, show_mismatches=true) (cigar.c:181-240):
- *                uint32 (len<<4)|op with op '='=7, X=8, I=1, D=2.
+ *   runs       : cigar_get_CIGAR(cigar, show_mismatches, ...) (cigar.c:181-240):
+ *                uint32 (len<<4)|op with op '='=7, X=8, I=1, D=2 (show_mismatches
+ *                true, the default), or M=0, I=1, D=2 with X folded into M
+ *                (false; wfa2ref_set_show_mismatches).
  */
 #include <stdbool.h>
 #include <stdint.h>
@@ -37,6 +39,9 @@
 #include "wavefront/wfa.h"
 #include "wavefront/wavefront_align.h"
 #include "alignment/cigar.h"
+
+static int g_show_mismatches = 1;
+void wfa2ref_set_show_mismatches(int show) { g_show_mismatches = show ? 1 : 0; }
 
 static double now_seconds(void) {
   struct timespec ts;
@@ -96,7 +101,7 @@ int wfa2ref_align_batch(
       if (runs != NULL) {
         int len = 0;
         uint32_t* buf = NULL;
-        if (!score_only && !empty) cigar_get_CIGAR(cg, true, &buf, &len);
+        if (!score_only && !empty) cigar_get_CIGAR(cg, g_show_mismatches != 0, &buf, &len);
         const int64_t cap = runs_off[i + 1] - runs_off[i];
         if (len > cap) {
           nruns[i] = -1;

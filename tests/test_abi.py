@@ -114,3 +114,30 @@ def test_runs_helpers():
     runs = [(14 << 4) | 7, (1 << 4) | 1, (86 << 4) | 7]
     assert bw.runs_to_cigar(runs) == "14M1I86M"
     assert bw.runs_to_ops(runs) == "M" * 14 + "I" + "M" * 86
+
+
+def build_c_stub(out_dir):
+    """Compiles tests/c_stub/alg_line.c (the integration stub) against the public
+    header and links it to the product library; returns the executable's path."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "c_stub", "alg_line.c")
+    exe = os.path.join(str(out_dir), "alg_line")
+    libdir = os.path.dirname(bw.LIB_PATH)
+    cmd = ["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(ROOT, "include"), src,
+           "-L" + libdir, "-lbiwfa_mi355x", "-Wl,-rpath," + libdir, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_c_stub_compiles_and_links_against_the_header(tmp_path):
+    exe = build_c_stub(tmp_path)
+    assert os.access(exe, os.X_OK)
+
+
+def test_integration_doc_shows_the_stub():
+    """The C stub in INTEGRATION.md is the file the tests compile."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    src = open(os.path.join(ROOT, "tests", "c_stub", "alg_line.c")).read()
+    body = src[src.index("int main"):]
+    assert body.strip() in doc

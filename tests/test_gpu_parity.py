@@ -113,6 +113,58 @@ def test_single_pair_api(aligners, utest_pairs):
         assert bw.runs_to_ops(a.sam_cigar()) == ops
 
 
+@pytest.mark.parametrize("tag", ["", ".p2"])
+def test_seq_file_to_alg_file(aligners, tmp_path, tag):
+    """align_benchmark's path end to end: the library reads wfa.utest.seq, aligns
+    every pair and writes the .alg file, which must equal WFA2's golden byte for
+    byte (full and score-only)."""
+    import os
+    from conftest import UTEST
+    pen = PENALTY_SETS[tag]
+    with bw.SeqFile(os.path.join(UTEST, "wfa.utest.seq")) as f:
+        packed = f.packed()
+    for score_only, name in ((False, "test.biwfa.affine%s.alg"), (True, "test.biwfa.score.affine%s.alg")):
+        name = name % tag
+        _say("seq -> alg %s" % name)
+        res = aligners(pen, score_only=score_only).align_batch(packed)
+        out = tmp_path / name
+        bw.write_alg(str(out), res, score_only=score_only)
+        assert out.read_bytes() == open(os.path.join(UTEST, name), "rb").read(), name
+
+
+@pytest.mark.parametrize("tag", ["", ".p5"])
+def test_single_pair_api_every_utest_pair(utest_pairs, oracle, tag):
+    """bw_align over all 305 utest pairs on one aligner (its device batch is reused
+    from call to call) against test.biwfa.affine<tag>.alg; then a pair longer than
+    the reserved capacity (the batch grows), a tuning change (the batch is planned
+    again) and the short pairs once more."""
+    import time
+    pen = PENALTY_SETS[tag]
+    gold = golden_alg(tag)
+    with bw.WavefrontAligner(penalties=pen) as a:
+        assert a.align(*utest_pairs[0]) == 0        # first call: reserves the batch
+        us = {False: [], True: []}
+        for i, (p, t) in enumerate(utest_pairs):
+            t0 = time.perf_counter()
+            assert a.align(p, t) == 0, "pair %d status %d" % (i, a.status)
+            us[len(p) + len(t) > 4000].append((time.perf_counter() - t0) * 1e6)
+            assert (a.score, a.cigar) == gold[i], "pair %d" % i
+        _say("bw_align%s: %.1f us/pair over all %d utest pairs; %.1f us/pair over the %d with p+t <= 4000, "
+             "%.1f over the %d ~10 kbp pairs (reused device batch; the first 10 kbp pair grows it)" % (
+                 tag, np.mean(us[False] + us[True]), len(utest_pairs), np.mean(us[False]), len(us[False]),
+                 np.mean(us[True]), len(us[True])))
+        rng = np.random.default_rng(7)
+        big_p = "".join(rng.choice(list("ACGT"), 3000))
+        big_t = big_p[:1000] + "".join(rng.choice(list("ACGT"), 40)) + big_p[1000:2500] + big_p[2600:]
+        r = oracle.align([(big_p, big_t)], penalties=pen)
+        assert a.align(big_p, big_t) == 0
+        assert a.score == int(r.alg_score[0]) and a.cigar == r.cigar(0)
+        a.tuning = {"team_waves": [1, 1, 2, 0, 0, 0, 0, 0]}
+        for i in (0, 150, 304):
+            assert a.align(*utest_pairs[i]) == 0 and (a.score, a.cigar) == gold[i]
+        assert a.align(big_p, big_t) == 0 and a.score == int(r.alg_score[0]) and a.cigar == r.cigar(0)
+
+
 def test_packed2bits_api(aligners, utest_pairs):
     gold = golden_alg("")
     a = aligners((0, 4, 6, 2))
@@ -424,3 +476,49 @@ def test_ready_guard_gives_up_and_launch_ends(oracle):
     ref = oracle.align(pairs, threads=16)
     _compare(res, ref.alg_score, [ref.cigar(i) for i in range(len(pairs))], "after-guard")
     a.close()
+
+
+def test_c_stub_prints_the_golden_alg_lines(tmp_path, utest_pairs):
+    """INTEGRATION.md's C stub, compiled against include/biwfa_mi355x.h and run as
+    its own process: its line for a utest pair equals WFA2's golden .alg line."""
+    import subprocess
+    from test_abi import build_c_stub
+    exe = build_c_stub(tmp_path)
+    gold = golden_alg("")
+    for i in (0, 1, 152, 304):
+        p, t = utest_pairs[i]
+        _say("c stub pair %d" % i)
+        r = subprocess.run([exe, p, t], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout == "%d\t%s\n" % gold[i]
+
+
+@pytest.mark.parametrize("tag", ["", ".p1", ".p4"])
+def test_sam_cigar_without_mismatches_matches_wfa2(aligners, utest_pairs, wfa2ref, tag):
+    """bw_get_sam_cigar(show_mismatches=0) against WFA2's own
+    cigar_get_CIGAR(cigar, false) (alignment/cigar.c:181-240), through oracle/_ref,
+    on every utest pair and on random pairs."""
+    pen = PENALTY_SETS[tag]
+    rng = np.random.default_rng(11)
+    pairs = list(utest_pairs)
+    for _ in range(40):
+        p = "".join(rng.choice(list("ACGT"), int(rng.integers(1, 400))))
+        t = list(p)
+        for _ in range(int(rng.integers(0, 30))):
+            j = int(rng.integers(0, len(t) + 1))
+            k = int(rng.integers(0, 3))
+            if k == 0 and j < len(t):
+                t[j] = str(rng.choice(list("ACGT")))
+            elif k == 1:
+                t.insert(j, str(rng.choice(list("ACGT"))))
+            elif j < len(t):
+                del t[j]
+        pairs.append((p, "".join(t) or "A"))
+    ref = wfa2ref.align(pairs, penalties=pen, show_mismatches=False)
+    ref_eq = wfa2ref.align(pairs, penalties=pen)
+    a = aligners(pen)
+    for i, (p, t) in enumerate(pairs):
+        assert a.align(p, t) == 0
+        got = a.sam_cigar(False).tolist()
+        assert got == [int(x) for x in ref.cigar_runs(i)], "pair %d" % i
+        assert a.sam_cigar(True).tolist() == [int(x) for x in ref_eq.cigar_runs(i)], "pair %d" % i

@@ -282,3 +282,34 @@ def test_oracle_c5_real_shape_vs_wfa2_golden(oracle):
         assert int(r.alg_score[i]) == score, i
         assert r.cigar(i) == cig, i
         assert int(r.wf_score[i]) == -meta["pairs"][i]["wfa2_cigar_score"], i
+
+
+def _sam_fold(runs):
+    """cigar_compute_CIGAR(show_mismatches=false) (alignment/cigar.c:181-229) restated
+    on show_mismatches=true runs: X becomes M (op 0) from the second operation on."""
+    ops = []
+    for r in runs:
+        ops += [int(r) & 15] * (int(r) >> 4)
+    out = []
+    for j, op in enumerate(ops):
+        if op == 7 or (op == 8 and j > 0):
+            op = 0
+        if out and out[-1][1] == op:
+            out[-1][0] += 1
+        else:
+            out.append([1, op])
+    return [(n << 4) | op for n, op in out]
+
+
+def test_sam_cigar_fold_rule_vs_wfa2(utest_pairs, wfa2ref):
+    """The leading-mismatch rule the product's bw_get_sam_cigar(show_mismatches=0)
+    follows, pinned against WFA2's own cigar_get_CIGAR(cigar, false)."""
+    pairs = list(utest_pairs) + [("ACGT", "TCGT"), ("AACGT", "TTCGT"), ("A", "C"), ("AC", "GT")]
+    full = wfa2ref.align(pairs)
+    folded = wfa2ref.align(pairs, show_mismatches=False)
+    lead_x = 0
+    for i in range(len(pairs)):
+        want = [int(x) for x in folded.cigar_runs(i)]
+        assert _sam_fold(full.cigar_runs(i)) == want, i
+        lead_x += bool(want) and (want[0] & 15) == 8
+    assert lead_x >= 3          # the rule is exercised
