@@ -44,7 +44,10 @@
 #define ST_END_REACHED -2
 #define ST_END_UNREACHABLE -3
 
-enum { CM = 0, CI = 1, CD = 2 };             /* components M, I1, D1 */
+/* components M, I1, D1, and for gap-affine-2p also I2, D2 (affine2p_matrix_type,
+ * affine2p_penalties.h) */
+enum { CM = 0, CI = 1, CD = 2, CI2 = 3, CD2 = 4 };
+#define NCOMP 5
 
 typedef struct {
   int present;     /* WFA2 keeps a non-NULL pointer for this component */
@@ -54,10 +57,12 @@ typedef struct {
   int cap;
 } wf_t;
 
-typedef struct { wf_t c[3]; } wfset_t;
+typedef struct { wf_t c[NCOMP]; } wfset_t;
 
 typedef struct {
   int x, o, e;                  /* internal penalties */
+  int o2, e2, a2p;              /* gap-affine-2p: second gap model (a2p = 1) */
+  int ncomp;                    /* 3, or 5 for gap-affine-2p */
   const char* P; const char* T; /* full sequences */
   int pb, plen, tb, tlen;       /* current sub-range */
   int reverse;                  /* aligner walks the sub-range backwards */
@@ -136,7 +141,7 @@ static const wf_t* fetch_input(aligner_t* a, int score, int comp) {
  * clear every wavefront and seed score 0 in the begin component at k=0. */
 static void aligner_init(aligner_t* a, int comp_begin, int comp_end) {
   for (int i = 0; i < a->nwfs; ++i)
-    for (int c = 0; c < 3; ++c) a->wfs[i].c[c].present = 0;
+    for (int c = 0; c < NCOMP; ++c) a->wfs[i].c[c].present = 0;
   a->comp_begin = comp_begin;
   a->comp_end = comp_end;
   a->null_steps = 0;
@@ -176,11 +181,87 @@ static void trim_ends(aligner_t* a, wf_t* w) {       /* wavefront_compute.c:579-
   w->lo = k;
 }
 
+/* wavefront_compute_affine2p (wavefront_compute_affine2p.c:45-108,334-367):
+ * seven inputs M[s-x], M[s-o1-e1], M[s-o2-e2], I1/D1[s-e1], I2/D2[s-e2]; the
+ * limits of wavefront_compute_limits_input (wavefront_compute.c:40-86) over all
+ * seven; I2/D2 are allocated when one of their inputs exists
+ * (wavefront_compute.c:474-493); M = max(del1, del2, misms, ins1, ins2), nulled
+ * when out of bounds; every output trimmed (wavefront_compute.c:615-632). */
+static void compute_affine2p(aligner_t* a, int s, wfset_t* out) {
+  const wf_t* mm = fetch_input(a, s - a->x, CM);
+  const wf_t* mo = fetch_input(a, s - a->o - a->e, CM);
+  const wf_t* mo2 = fetch_input(a, s - a->o2 - a->e2, CM);
+  const wf_t* ie = fetch_input(a, s - a->e, CI);
+  const wf_t* de = fetch_input(a, s - a->e, CD);
+  const wf_t* ie2 = fetch_input(a, s - a->e2, CI2);
+  const wf_t* de2 = fetch_input(a, s - a->e2, CD2);
+  if (!mm && !mo && !mo2 && !ie && !de && !ie2 && !de2) {   /* null step (:346-357) */
+    a->null_steps++;
+    return;
+  }
+  a->null_steps = 0;
+  int lo = mm ? mm->lo : 1, hi = mm ? mm->hi : -1, t;
+  t = (mo ? mo->lo : 1) - 1;   if (lo > t) lo = t;
+  t = (mo ? mo->hi : -1) + 1;  if (hi < t) hi = t;
+  t = (ie ? ie->lo : 1) + 1;   if (lo > t) lo = t;
+  t = (ie ? ie->hi : -1) + 1;  if (hi < t) hi = t;
+  t = (de ? de->lo : 1) - 1;   if (lo > t) lo = t;
+  t = (de ? de->hi : -1) - 1;  if (hi < t) hi = t;
+  t = (mo2 ? mo2->lo : 1) - 1; if (lo > t) lo = t;
+  t = (mo2 ? mo2->hi : -1) + 1; if (hi < t) hi = t;
+  t = (ie2 ? ie2->lo : 1) + 1; if (lo > t) lo = t;
+  t = (ie2 ? ie2->hi : -1) + 1; if (hi < t) hi = t;
+  t = (de2 ? de2->lo : 1) - 1; if (lo > t) lo = t;
+  t = (de2 ? de2->hi : -1) - 1; if (hi < t) hi = t;
+  a->cnt->cells += hi - lo + 1;
+  a->cnt->compute_steps++;
+  const int has_i = (mo != NULL || ie != NULL), has_d = (mo != NULL || de != NULL);
+  const int has_i2 = (mo2 != NULL || ie2 != NULL), has_d2 = (mo2 != NULL || de2 != NULL);
+  wf_t* om = &out->c[CM];
+  wf_t* oi = &out->c[CI];
+  wf_t* od = &out->c[CD];
+  wf_t* oi2 = &out->c[CI2];
+  wf_t* od2 = &out->c[CD2];
+  wf_reserve(om, lo, hi);
+  if (has_i) wf_reserve(oi, lo, hi);
+  if (has_d) wf_reserve(od, lo, hi);
+  if (has_i2) wf_reserve(oi2, lo, hi);
+  if (has_d2) wf_reserve(od2, lo, hi);
+  const uint32_t tl = (uint32_t)a->tlen, pl = (uint32_t)a->plen;
+  for (int k = lo; k <= hi; ++k) {
+    int32_t a1 = wf_get(mo, k - 1), b1 = wf_get(ie, k - 1);
+    const int32_t ins1 = (a1 >= b1 ? a1 : b1) + 1;
+    a1 = wf_get(mo2, k - 1); b1 = wf_get(ie2, k - 1);
+    const int32_t ins2 = (a1 >= b1 ? a1 : b1) + 1;
+    const int32_t ins = ins1 >= ins2 ? ins1 : ins2;
+    a1 = wf_get(mo, k + 1); b1 = wf_get(de, k + 1);
+    const int32_t del1 = (a1 >= b1 ? a1 : b1);
+    a1 = wf_get(mo2, k + 1); b1 = wf_get(de2, k + 1);
+    const int32_t del2 = (a1 >= b1 ? a1 : b1);
+    const int32_t del = del1 >= del2 ? del1 : del2;
+    const int32_t misms = wf_get(mm, k) + 1;
+    int32_t mx = (misms >= ins ? misms : ins);
+    mx = (del >= mx ? del : mx);
+    if ((uint32_t)mx > tl || (uint32_t)(mx - k) > pl) mx = OFFSET_NULL;
+    om->off[k - lo] = mx;
+    if (has_i) oi->off[k - lo] = ins1;
+    if (has_d) od->off[k - lo] = del1;
+    if (has_i2) oi2->off[k - lo] = ins2;
+    if (has_d2) od2->off[k - lo] = del2;
+  }
+  trim_ends(a, om);
+  if (has_i) trim_ends(a, oi);
+  if (has_d) trim_ends(a, od);
+  if (has_i2) trim_ends(a, oi2);
+  if (has_d2) trim_ends(a, od2);
+}
+
 static void compute_affine(aligner_t* a, int s) {
   /* Reserve the output slot first: growing the slot table may move it.  The
    * inputs (scores s-x, s-o-e, s-e) never share the ring slot of s. */
   wfset_t* out = slot_of(a, s);
-  out->c[CM].present = out->c[CI].present = out->c[CD].present = 0;
+  for (int q = 0; q < NCOMP; ++q) out->c[q].present = 0;
+  if (a->a2p) { compute_affine2p(a, s, out); return; }
   const wf_t* mm = fetch_input(a, s - a->x, CM);
   const wf_t* mo = fetch_input(a, s - a->o - a->e, CM);
   const wf_t* ie = fetch_input(a, s - a->e, CI);
@@ -360,6 +441,13 @@ static void bialign_overlap(aligner_t* a0, aligner_t* a1, int s0, int s1, int fo
     const int si = s1 - i;
     if (si < 0) break;
     wfset_t* set1 = slot_of(a1, si);
+    if (a0->a2p) {                   /* I2/D2 breakpoints first (:347-364) */
+      if (s0 + si - a0->o2 >= bp->score) continue;
+      if (set0->c[CD2].present && set1->c[CD2].present)
+        breakpoint_indel2indel(a0, forward, s0, si, &set0->c[CD2], &set1->c[CD2], CD2, a0->o2, bp);
+      if (set0->c[CI2].present && set1->c[CI2].present)
+        breakpoint_indel2indel(a0, forward, s0, si, &set0->c[CI2], &set1->c[CI2], CI2, a0->o2, bp);
+    }
     if (s0 + si - gap_open >= bp->score) continue;
     if (set0->c[CD].present && set1->c[CD].present)
       breakpoint_indel2indel(a0, forward, s0, si, &set0->c[CD], &set1->c[CD], CD, gap_open, bp);
@@ -400,7 +488,9 @@ static int find_breakpoint(ctx_t* c, int comp_begin, int comp_end, breakpoint_t*
     if (rq) return r->status;
     if (sr + sf >= c->max_steps) return BWO_STATUS_MAX_STEPS;
   }
-  const int scope = f->scope, gap_open = f->o;               /* phase 2 (:482-516) */
+  /* phase 2 (:482-516); the opening penalty of the overlap bound is
+   * wavefront_bialign_overlap_gopen_adjust (:396-410): max(o1, o2) for 2p */
+  const int scope = f->scope, gap_open = (f->a2p && f->o2 > f->o) ? f->o2 : f->o;
   for (;;) {
     if (last_forward) {
       const int min_r = (sr > scope - 1) ? sr - (scope - 1) : 0;
@@ -453,10 +543,14 @@ static void rops_push(ctx_t* c, int* n, char op, int cnt) {
 }
 
 #define BT_M 9
+#define BT_D2_EXT 8
+#define BT_D2_OPEN 7
 #define BT_D1_EXT 6
 #define BT_D1_OPEN 5
 #define BT_I1_EXT 2
 #define BT_I1_OPEN 1
+#define BT_I2_EXT 4
+#define BT_I2_OPEN 3
 
 /* wavefront_backtrace_* source lookups (wavefront_backtrace.c:64-219): the
  * candidate is the packed (offset << 4) | type, or OFFSET_NULL itself. */
@@ -481,6 +575,8 @@ static int backtrace_affine(ctx_t* c, aligner_t* a, int score) {
     const int mismatch = score - a->x;
     const int gap_open1 = score - a->o - a->e;
     const int gap_extend1 = score - a->e;
+    const int gap_open2 = score - a->o2 - a->e2;
+    const int gap_extend2 = score - a->e2;
     int64_t max_all;
     if (matrix == CM) {
       const int64_t misms = bt_src(a, mismatch, CM, k, 1, BT_M);
@@ -489,12 +585,25 @@ static int backtrace_affine(ctx_t* c, aligner_t* a, int score) {
       const int64_t dop = bt_src(a, gap_open1, CM, k + 1, 0, BT_D1_OPEN);
       const int64_t dx = bt_src(a, gap_extend1, CD, k + 1, 0, BT_D1_EXT);
       max_all = max64(misms, max64(max64(io, ix), max64(dop, dx)));
+      if (a->a2p) {                  /* the I2/D2 sources (wavefront_backtrace.c:377-386) */
+        const int64_t io2 = bt_src(a, gap_open2, CM, k - 1, 1, BT_I2_OPEN);
+        const int64_t ix2 = bt_src(a, gap_extend2, CI2, k - 1, 1, BT_I2_EXT);
+        const int64_t dop2 = bt_src(a, gap_open2, CM, k + 1, 0, BT_D2_OPEN);
+        const int64_t dx2 = bt_src(a, gap_extend2, CD2, k + 1, 0, BT_D2_EXT);
+        max_all = max64(max_all, max64(max64(io2, ix2), max64(dop2, dx2)));
+      }
     } else if (matrix == CI) {
       max_all = max64(bt_src(a, gap_open1, CM, k - 1, 1, BT_I1_OPEN),
                       bt_src(a, gap_extend1, CI, k - 1, 1, BT_I1_EXT));
-    } else {
+    } else if (matrix == CD) {
       max_all = max64(bt_src(a, gap_open1, CM, k + 1, 0, BT_D1_OPEN),
                       bt_src(a, gap_extend1, CD, k + 1, 0, BT_D1_EXT));
+    } else if (matrix == CI2) {
+      max_all = max64(bt_src(a, gap_open2, CM, k - 1, 1, BT_I2_OPEN),
+                      bt_src(a, gap_extend2, CI2, k - 1, 1, BT_I2_EXT));
+    } else {
+      max_all = max64(bt_src(a, gap_open2, CM, k + 1, 0, BT_D2_OPEN),
+                      bt_src(a, gap_extend2, CD2, k + 1, 0, BT_D2_EXT));
     }
     if (max_all < 0) break;
     if (matrix == CM) {
@@ -510,6 +619,10 @@ static int backtrace_affine(ctx_t* c, aligner_t* a, int score) {
       case BT_I1_EXT:  score = gap_extend1; matrix = CI; rops_push(c, &n, 'I', 1); --k; --off; break;
       case BT_D1_OPEN: score = gap_open1;   matrix = CM; rops_push(c, &n, 'D', 1); ++k; break;
       case BT_D1_EXT:  score = gap_extend1; matrix = CD; rops_push(c, &n, 'D', 1); ++k; break;
+      case BT_I2_OPEN: score = gap_open2;   matrix = CM; rops_push(c, &n, 'I', 1); --k; --off; break;
+      case BT_I2_EXT:  score = gap_extend2; matrix = CI2; rops_push(c, &n, 'I', 1); --k; --off; break;
+      case BT_D2_OPEN: score = gap_open2;   matrix = CM; rops_push(c, &n, 'D', 1); ++k; break;
+      case BT_D2_EXT:  score = gap_extend2; matrix = CD2; rops_push(c, &n, 'D', 1); ++k; break;
       default: return -1;
     }
     v = off - k; h = off;
@@ -594,18 +707,31 @@ static int bialign_alignment(ctx_t* c, int pb, int pe, int tb, int te,
 /* ------------------------------------------------------------------------- */
 /* driver                                                                    */
 
+/* User penalties: gap-affine (a2p = 0) or gap-affine-2p (a2p = 1, second gap
+ * model o2 + e2 * length). */
+typedef struct { int match, x, o, e, o2, e2, a2p; } pen_t;
+
 static void ctx_setup(ctx_t* c, const char* P, int plen, const char* T, int tlen,
-                      int match, int x, int o, int e, int max_steps) {
-  /* wavefront_penalties_set_affine (wavefront_penalties.c:95-131) */
-  int ix = x, io = o, ie = e;
-  if (match < 0) { ix = 2 * x - 2 * match; io = 2 * o; ie = 2 * e - match; }
-  /* wavefront_components_dimensions_affine (wavefront_components.c:81-100) */
-  const int scope_indel = io + ie;
+                      const pen_t* u, int max_steps) {
+  /* wavefront_penalties_set_affine (wavefront_penalties.c:95-131) and
+   * wavefront_penalties_set_affine2p (:132-173): Eizenga's transform for match < 0 */
+  const int match = u->match;
+  int ix = u->x, io = u->o, ie = u->e, io2 = u->o2, ie2 = u->e2;
+  if (match < 0) {
+    ix = 2 * u->x - 2 * match; io = 2 * u->o; ie = 2 * u->e - match;
+    io2 = 2 * u->o2; ie2 = 2 * u->e2 - match;
+  }
+  /* wavefront_components_dimensions_affine / _affine2p (wavefront_components.c:81-112) */
+  int scope_indel = io + ie;
+  if (u->a2p && io2 + ie2 > scope_indel) scope_indel = io2 + ie2;
   const int scope = (scope_indel > ix ? scope_indel : ix) + 1;
   aligner_t* as[3] = {&c->fwd, &c->rev, &c->base};
   for (int i = 0; i < 3; ++i) {
     aligner_t* a = as[i];
     a->x = ix; a->o = io; a->e = ie;
+    a->a2p = u->a2p;
+    a->o2 = u->a2p ? io2 : 0; a->e2 = u->a2p ? ie2 : 0;
+    a->ncomp = u->a2p ? 5 : 3;
     a->P = P; a->T = T;
     a->scope = scope;
     a->modular = (i < 2);          /* breakpoint aligners are score-only (modular) */
@@ -624,7 +750,7 @@ static void ctx_free(ctx_t* c) {
   aligner_t* as[3] = {&c->fwd, &c->rev, &c->base};
   for (int i = 0; i < 3; ++i) {
     for (int s = 0; s < as[i]->nwfs; ++s)
-      for (int q = 0; q < 3; ++q) free(as[i]->wfs[s].c[q].off);
+      for (int q = 0; q < NCOMP; ++q) free(as[i]->wfs[s].c[q].off);
     free(as[i]->wfs);
   }
   free(c->ops);
@@ -647,6 +773,27 @@ static int score_gap_affine(const char* ops, int n, int match, int x, int o, int
   return score;
 }
 
+/* cigar_score_gap_affine2p (cigar.c:304-342): each maximal gap run costs the
+ * cheaper of its two gap models */
+static int score_gap_affine2p(const char* ops, int n, const pen_t* u) {
+  int score = 0;
+  for (int i = 0; i < n;) {
+    int j = i;
+    while (j < n && ops[j] == ops[i]) ++j;
+    const int len = j - i;
+    switch (ops[i]) {
+      case 'M': score -= u->match * len; break;
+      case 'X': score -= u->x * len; break;
+      default: {
+        const int s1 = u->o + u->e * len, s2 = u->o2 + u->e2 * len;
+        score -= s1 < s2 ? s1 : s2;
+      }
+    }
+    i = j;
+  }
+  return score;
+}
+
 static int ops_to_runs(const char* ops, int n, uint32_t* runs, int cap) {
   int nr = 0;
   for (int i = 0; i < n;) {
@@ -661,9 +808,10 @@ static int ops_to_runs(const char* ops, int n, uint32_t* runs, int cap) {
 }
 
 static int align_ctx(ctx_t* c, const char* P, int plen, const char* T, int tlen,
-                     int match, int x, int o, int e, int score_only, int max_steps,
+                     const pen_t* u, int score_only, int max_steps,
                      uint32_t* runs, int runs_cap, int* nruns, int* alg_score, int* wf_score) {
-  ctx_setup(c, P, plen, T, tlen, match, x, o, e, max_steps);
+  const int match = u->match;
+  ctx_setup(c, P, plen, T, tlen, u, max_steps);
   if (nruns) *nruns = 0;
   if (score_only) {
     /* wavefront_bialign_compute_score (wavefront_bialign.c:662-699) */
@@ -691,7 +839,9 @@ static int align_ctx(ctx_t* c, const char* P, int plen, const char* T, int tlen,
   else if (st == BWO_STATUS_MAX_STEPS || st == BWO_STATUS_OOM) status = st;
   else status = BWO_STATUS_UNATTAIN;
   *wf_score = c->level0_wf_score;
-  *alg_score = (c->nops > 0) ? score_gap_affine(c->ops, c->nops, match, x, o, e) : -1;
+  if (c->nops <= 0) *alg_score = -1;
+  else if (u->a2p) *alg_score = score_gap_affine2p(c->ops, c->nops, u);
+  else *alg_score = score_gap_affine(c->ops, c->nops, match, u->x, u->o, u->e);
   if (runs) {
     const int nr = ops_to_runs(c->ops, c->nops, runs, runs_cap);
     if (nr < 0) { if (nruns) *nruns = -1; return BWO_STATUS_CAPACITY; }
@@ -706,7 +856,8 @@ int bwo_align(const char* pattern, int plen, const char* text, int tlen,
               int* alg_score, int* wf_score, bwo_counters_t* counters) {
   ctx_t c;
   memset(&c, 0, sizeof(c));
-  const int st = align_ctx(&c, pattern, plen, text, tlen, match, x, o, e, score_only,
+  const pen_t u = {match, x, o, e, 0, 0, 0};
+  const int st = align_ctx(&c, pattern, plen, text, tlen, &u, score_only,
                            max_steps, runs, runs_cap, nruns, alg_score, wf_score);
   if (counters) *counters = c.cnt;
   ctx_free(&c);
@@ -719,14 +870,13 @@ static double now_seconds(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-int bwo_align_batch(int n, const char* seqbuf,
-                    const int64_t* p_off, const int32_t* p_len,
-                    const int64_t* t_off, const int32_t* t_len,
-                    int match, int x, int o, int e, int score_only, int max_steps,
-                    int nthreads,
-                    int32_t* alg_score, int32_t* wf_score, int32_t* status,
-                    uint32_t* runs, const int64_t* runs_off, int32_t* nruns,
-                    int64_t* cells, int64_t* words, double* elapsed) {
+static void align_batch(int n, const char* seqbuf,
+                        const int64_t* p_off, const int32_t* p_len,
+                        const int64_t* t_off, const int32_t* t_len,
+                        const pen_t* u, int score_only, int max_steps, int nthreads,
+                        int32_t* alg_score, int32_t* wf_score, int32_t* status,
+                        uint32_t* runs, const int64_t* runs_off, int32_t* nruns,
+                        int64_t* cells, int64_t* words, double* elapsed) {
   if (nthreads <= 0) nthreads = omp_get_max_threads();
   const double t0 = now_seconds();
   #pragma omp parallel num_threads(nthreads)
@@ -739,7 +889,7 @@ int bwo_align_batch(int n, const char* seqbuf,
       int nr = 0, as = 0, ws = 0;
       const int64_t cap = runs ? runs_off[i + 1] - runs_off[i] : 0;
       status[i] = align_ctx(&c, seqbuf + p_off[i], p_len[i], seqbuf + t_off[i], t_len[i],
-                            match, x, o, e, score_only, max_steps,
+                            u, score_only, max_steps,
                             runs ? runs + runs_off[i] : NULL, (int)cap, &nr, &as, &ws);
       alg_score[i] = as;
       wf_score[i] = ws;
@@ -750,5 +900,32 @@ int bwo_align_batch(int n, const char* seqbuf,
     ctx_free(&c);
   }
   if (elapsed) *elapsed = now_seconds() - t0;
+}
+
+int bwo_align_batch(int n, const char* seqbuf,
+                    const int64_t* p_off, const int32_t* p_len,
+                    const int64_t* t_off, const int32_t* t_len,
+                    int match, int x, int o, int e, int score_only, int max_steps,
+                    int nthreads,
+                    int32_t* alg_score, int32_t* wf_score, int32_t* status,
+                    uint32_t* runs, const int64_t* runs_off, int32_t* nruns,
+                    int64_t* cells, int64_t* words, double* elapsed) {
+  const pen_t u = {match, x, o, e, 0, 0, 0};
+  align_batch(n, seqbuf, p_off, p_len, t_off, t_len, &u, score_only, max_steps, nthreads,
+              alg_score, wf_score, status, runs, runs_off, nruns, cells, words, elapsed);
+  return 0;
+}
+
+int bwo_align_batch_2p(int n, const char* seqbuf,
+                       const int64_t* p_off, const int32_t* p_len,
+                       const int64_t* t_off, const int32_t* t_len,
+                       int match, int x, int o1, int e1, int o2, int e2,
+                       int score_only, int max_steps, int nthreads,
+                       int32_t* alg_score, int32_t* wf_score, int32_t* status,
+                       uint32_t* runs, const int64_t* runs_off, int32_t* nruns,
+                       int64_t* cells, int64_t* words, double* elapsed) {
+  const pen_t u = {match, x, o1, e1, o2, e2, 1};
+  align_batch(n, seqbuf, p_off, p_len, t_off, t_len, &u, score_only, max_steps, nthreads,
+              alg_score, wf_score, status, runs, runs_off, nruns, cells, words, elapsed);
   return 0;
 }

@@ -1,6 +1,7 @@
 /*
  * biwfa_oracle.h -- CPU restatement of WFA2-lib v2.3's bidirectional WFA
- * (memory_mode = ultralow, heuristic = none, gap-affine), score + CIGAR.
+ * (memory_mode = ultralow, heuristic = none, gap-affine and gap-affine-2p),
+ * score + CIGAR.
  *
  * TEST INFRASTRUCTURE ONLY.  Used by tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg as the checker; the product library never links it.
@@ -52,6 +53,17 @@ int bwo_align_batch(int n, const char* seqbuf,
                     int32_t* alg_score, int32_t* wf_score, int32_t* status,
                     uint32_t* runs, const int64_t* runs_off, int32_t* nruns,
                     int64_t* cells, int64_t* words, double* elapsed);
+
+/* The same under gap-affine-2p penalties (match, x, o1, e1, o2, e2): gap runs
+ * cost min(o1 + e1*len, o2 + e2*len) (WFA2 gap_affine_2p). */
+int bwo_align_batch_2p(int n, const char* seqbuf,
+                       const int64_t* p_off, const int32_t* p_len,
+                       const int64_t* t_off, const int32_t* t_len,
+                       int match, int x, int o1, int e1, int o2, int e2,
+                       int score_only, int max_steps, int nthreads,
+                       int32_t* alg_score, int32_t* wf_score, int32_t* status,
+                       uint32_t* runs, const int64_t* runs_off, int32_t* nruns,
+                       int64_t* cells, int64_t* words, double* elapsed);
 
 #ifdef __cplusplus
 }

@@ -126,11 +126,17 @@ class _Engine:
     SYMBOL = None
 
     def align(self, pairs, penalties=(0, 4, 6, 2), score_only=False, threads=0, max_steps=0):
-        """pairs: [(pattern, text), ...] or (ascii_buf, p_off, p_len, t_off, t_len)."""
+        """pairs: [(pattern, text), ...] or (ascii_buf, p_off, p_len, t_off, t_len).
+        penalties: (match, x, o, e) gap-affine, or (match, x, o1, e1, o2, e2) gap-affine-2p."""
         buf, p_off, p_len, t_off, t_len = _prep(pairs)
         n = len(p_len)
         res = _Result(n, (p_len.astype(np.int64) + t_len + 1))
         el = ctypes.c_double(0.0)
+        if len(penalties) == 6:
+            self._call2p(n, buf, p_off, p_len, t_off, t_len, tuple(int(v) for v in penalties), score_only,
+                         max_steps, threads, res, el)
+            res.elapsed = el.value
+            return res
         m, x, o, e = penalties
         self._call(n, buf, p_off, p_len, t_off, t_len, m, x, o, e, score_only, max_steps,
                    threads, res, el)
@@ -177,6 +183,17 @@ class Wfa2Ref(_Engine):
             _p(res.runs_off, ctypes.c_int64), _p(res.nruns, ctypes.c_int32),
             ctypes.byref(el))
 
+    def _call2p(self, n, buf, p_off, p_len, t_off, t_len, pen, score_only, max_steps, threads, res, el):
+        self.lib.wfa2ref_align_batch_2p(
+            ctypes.c_int(n), buf.ctypes.data_as(ctypes.c_char_p),
+            _p(p_off, ctypes.c_int64), _p(p_len, ctypes.c_int32),
+            _p(t_off, ctypes.c_int64), _p(t_len, ctypes.c_int32),
+            *pen, int(score_only), int(max_steps), int(threads),
+            _p(res.alg_score, ctypes.c_int32), _p(res.raw_score, ctypes.c_int32),
+            _p(res.status, ctypes.c_int32), _p(res.runs, ctypes.c_uint32),
+            _p(res.runs_off, ctypes.c_int64), _p(res.nruns, ctypes.c_int32),
+            ctypes.byref(el))
+
 
 class Oracle(_Engine):
     """The C restatement of WFA2-lib's BiWFA (oracle/biwfa_oracle.c)."""
@@ -206,6 +223,21 @@ class Oracle(_Engine):
             _p(p_off, ctypes.c_int64), _p(p_len, ctypes.c_int32),
             _p(t_off, ctypes.c_int64), _p(t_len, ctypes.c_int32),
             m, x, o, e, int(score_only), int(max_steps), int(threads),
+            _p(res.alg_score, ctypes.c_int32), _p(res.wf_score, ctypes.c_int32),
+            _p(res.status, ctypes.c_int32), _p(res.runs, ctypes.c_uint32),
+            _p(res.runs_off, ctypes.c_int64), _p(res.nruns, ctypes.c_int32),
+            _p(res.cells, ctypes.c_int64), _p(res.words, ctypes.c_int64),
+            ctypes.byref(el))
+
+    def _call2p(self, n, buf, p_off, p_len, t_off, t_len, pen, score_only, max_steps, threads, res, el):
+        res.wf_score = res.raw_score
+        res.cells = np.zeros(n, np.int64)
+        res.words = np.zeros(n, np.int64)
+        self.lib.bwo_align_batch_2p(
+            ctypes.c_int(n), buf.ctypes.data_as(ctypes.c_char_p),
+            _p(p_off, ctypes.c_int64), _p(p_len, ctypes.c_int32),
+            _p(t_off, ctypes.c_int64), _p(t_len, ctypes.c_int32),
+            *pen, int(score_only), int(max_steps), int(threads),
             _p(res.alg_score, ctypes.c_int32), _p(res.wf_score, ctypes.c_int32),
             _p(res.status, ctypes.c_int32), _p(res.runs, ctypes.c_uint32),
             _p(res.runs_off, ctypes.c_int64), _p(res.nruns, ctypes.c_int32),

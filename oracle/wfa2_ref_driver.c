@@ -10,7 +10,7 @@
  *
  * Aligner settings are the exact-BiWFA ones (the library defaults are the
  * WF-adaptive heuristic in high-memory mode, wavefront_attributes.c:71-78):
- *   distance_metric = gap_affine, heuristic = none,
+ *   distance_metric = gap_affine (or gap_affine_2p), heuristic = none,
  *   memory_mode = ultralow (BiWFA, wavefront_bialign.c:703),
  *   alignment_scope = compute_alignment (or compute_score).
  * One aligner per OpenMP thread, reused across pairs, as in
@@ -19,7 +19,7 @@
  * Outputs per pair:
  *   alg_score  : the score column align_benchmark writes to .alg files
  *                (tools/align_benchmark/benchmark/benchmark_utils.c:143-179): cigar_score_gap_affine
- *                of the CIGAR for full alignments, cigar->score for score-only,
+ *                (cigar_score_gap_affine2p) of the CIGAR for full alignments, cigar->score for score-only,
  *                -1 when the CIGAR is empty.
  *   raw_score  : aligner->cigar->score as left by wavefront_align.
  *   status     : wavefront_align's return code (wfa.h:46-55).
@@ -49,14 +49,19 @@ static double now_seconds(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-static wavefront_aligner_t* new_aligner(int match, int x, int o, int e,
+static wavefront_aligner_t* new_aligner(const affine2p_penalties_t* p, int two_piece,
                                         int score_only, int max_steps) {
   wavefront_aligner_attr_t attr = wavefront_aligner_attr_default;
-  attr.distance_metric = gap_affine;
-  attr.affine_penalties.match = match;
-  attr.affine_penalties.mismatch = x;
-  attr.affine_penalties.gap_opening = o;
-  attr.affine_penalties.gap_extension = e;
+  if (two_piece) {
+    attr.distance_metric = gap_affine_2p;
+    attr.affine2p_penalties = *p;
+  } else {
+    attr.distance_metric = gap_affine;
+    attr.affine_penalties.match = p->match;
+    attr.affine_penalties.mismatch = p->mismatch;
+    attr.affine_penalties.gap_opening = p->gap_opening1;
+    attr.affine_penalties.gap_extension = p->gap_extension1;
+  }
   attr.heuristic.strategy = wf_heuristic_none;
   attr.memory_mode = wavefront_memory_ultralow;
   attr.alignment_scope = score_only ? compute_score : compute_alignment;
@@ -70,11 +75,11 @@ static wavefront_aligner_t* new_aligner(int match, int x, int o, int e,
  * the number of runs (or -1 when the capacity was too small).
  * Returns the wall time of the parallel region in *elapsed (seconds).
  */
-int wfa2ref_align_batch(
+static void align_batch(
     int n, const char* seqbuf,
     const int64_t* p_off, const int32_t* p_len,
     const int64_t* t_off, const int32_t* t_len,
-    int match, int x, int o, int e, int score_only, int max_steps, int nthreads,
+    const affine2p_penalties_t* pen2, int two_piece, int score_only, int max_steps, int nthreads,
     int32_t* alg_score, int32_t* raw_score, int32_t* status,
     uint32_t* runs, const int64_t* runs_off, int32_t* nruns,
     double* elapsed) {
@@ -82,9 +87,9 @@ int wfa2ref_align_batch(
   const double t0 = now_seconds();
   #pragma omp parallel num_threads(nthreads)
   {
-    wavefront_aligner_t* const wf = new_aligner(match, x, o, e, score_only, max_steps);
-    affine_penalties_t pen = {.match = match, .mismatch = x,
-                              .gap_opening = o, .gap_extension = e};
+    wavefront_aligner_t* const wf = new_aligner(pen2, two_piece, score_only, max_steps);
+    affine_penalties_t pen = {.match = pen2->match, .mismatch = pen2->mismatch,
+                              .gap_opening = pen2->gap_opening1, .gap_extension = pen2->gap_extension1};
     #pragma omp for schedule(dynamic, 16)
     for (int i = 0; i < n; ++i) {
       const int st = wavefront_align(wf, seqbuf + p_off[i], p_len[i],
@@ -96,7 +101,8 @@ int wfa2ref_align_batch(
       if (score_only) {
         alg_score[i] = cg->score;
       } else {
-        alg_score[i] = empty ? -1 : cigar_score_gap_affine(cg, &pen);
+        alg_score[i] = empty ? -1 : two_piece ? cigar_score_gap_affine2p(cg, pen2)
+                                              : cigar_score_gap_affine(cg, &pen);
       }
       if (runs != NULL) {
         int len = 0;
@@ -114,6 +120,35 @@ int wfa2ref_align_batch(
     wavefront_aligner_delete(wf);
   }
   if (elapsed) *elapsed = now_seconds() - t0;
+}
+
+int wfa2ref_align_batch(
+    int n, const char* seqbuf,
+    const int64_t* p_off, const int32_t* p_len,
+    const int64_t* t_off, const int32_t* t_len,
+    int match, int x, int o, int e, int score_only, int max_steps, int nthreads,
+    int32_t* alg_score, int32_t* raw_score, int32_t* status,
+    uint32_t* runs, const int64_t* runs_off, int32_t* nruns,
+    double* elapsed) {
+  const affine2p_penalties_t p = {.match = match, .mismatch = x, .gap_opening1 = o,
+                                  .gap_extension1 = e, .gap_opening2 = 0, .gap_extension2 = 0};
+  align_batch(n, seqbuf, p_off, p_len, t_off, t_len, &p, 0, score_only, max_steps, nthreads,
+              alg_score, raw_score, status, runs, runs_off, nruns, elapsed);
+  return 0;
+}
+
+/* The same under gap_affine_2p (penalties match, x, o1, e1, o2, e2). */
+int wfa2ref_align_batch_2p(
+    int n, const char* seqbuf,
+    const int64_t* p_off, const int32_t* p_len,
+    const int64_t* t_off, const int32_t* t_len,
+    int match, int x, int o1, int e1, int o2, int e2, int score_only, int max_steps,
+    int nthreads, int32_t* alg_score, int32_t* raw_score, int32_t* status,
+    uint32_t* runs, const int64_t* runs_off, int32_t* nruns, double* elapsed) {
+  const affine2p_penalties_t p = {.match = match, .mismatch = x, .gap_opening1 = o1,
+                                  .gap_extension1 = e1, .gap_opening2 = o2, .gap_extension2 = e2};
+  align_batch(n, seqbuf, p_off, p_len, t_off, t_len, &p, 1, score_only, max_steps, nthreads,
+              alg_score, raw_score, status, runs, runs_off, nruns, elapsed);
   return 0;
 }
 

@@ -24,6 +24,11 @@ PENALTY_SETS = {
 }
 
 
+# Gap-affine-2p penalties of WFA2's utest (align_benchmark's defaults,
+# tools/align_benchmark/align_benchmark_params.c:62-69): (match, x, o1, e1, o2, e2)
+AFFINE2P = (0, 4, 6, 2, 24, 1)
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
     config.addinivalue_line("markers", "slow: CPU checks of several minutes; run with BIWFA_RUN_SLOW=1")
@@ -75,4 +80,11 @@ def utest_pairs():
 def golden_alg(tag, score_only=False):
     from oracle.oracle import read_alg_file
     name = "test.biwfa.%saffine%s.alg" % ("score." if score_only else "", tag)
+    return read_alg_file(os.path.join(UTEST, name))
+
+
+def golden_alg2p(score_only=False):
+    """WFA2's BiWFA goldens under gap-affine-2p (tests/wfa.utest.check/test.biwfa[.score].affine2p.alg)."""
+    from oracle.oracle import read_alg_file
+    name = "test.biwfa.%saffine2p.alg" % ("score." if score_only else "")
     return read_alg_file(os.path.join(UTEST, name))

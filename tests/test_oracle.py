@@ -9,11 +9,17 @@
   and rescore to the reported score (cigar_check_alignment / cigar_score_gap_affine,
   alignment/cigar.c:277-303, 617-688);
 * a planted mistake in the oracle (a wrong tie-break) must fail the golden check.
+
+Gap-affine-2p (five components M, I1, D1, I2, D2) is pinned the same way: the
+fixture's 2p goldens (full and score-only), the compiled WFA2-lib on random and
+long-indel pairs under several 2p penalty sets (match < 0 and o2 < o1 included)
+and under step caps, a two-piece Gotoh DP for the optimum of small pairs, and a
+planted mistake (M without the I2 input) that the goldens catch.
 """
 import numpy as np
 import pytest
 
-from conftest import PENALTY_SETS, golden_alg
+from conftest import AFFINE2P, PENALTY_SETS, golden_alg, golden_alg2p
 from oracle.oracle import runs_to_cigar
 
 
@@ -313,3 +319,148 @@ def test_sam_cigar_fold_rule_vs_wfa2(utest_pairs, wfa2ref):
         assert _sam_fold(full.cigar_runs(i)) == want, i
         lead_x += bool(want) and (want[0] & 15) == 8
     assert lead_x >= 3          # the rule is exercised
+
+
+# ---------------------------------------------------------------------------
+# gap-affine-2p
+
+AFFINE2P_SETS = [AFFINE2P, (0, 4, 8, 2, 4, 3), (0, 3, 24, 1, 6, 2), (-2, 4, 6, 2, 24, 1),
+                 (-1, 1, 2, 1, 5, 1), (0, 5, 2, 3, 10, 1)]
+
+
+def gotoh2p_cost(p, t, match, x, o1, e1, o2, e2):
+    """Two-piece Gotoh DP (minimum cost; every gap run pays the cheaper model)."""
+    INF = 1 << 40
+    n, m = len(p), len(t)
+    M = [[INF] * (m + 1) for _ in range(n + 1)]
+    I1 = [[INF] * (m + 1) for _ in range(n + 1)]
+    I2 = [[INF] * (m + 1) for _ in range(n + 1)]
+    D1 = [[INF] * (m + 1) for _ in range(n + 1)]
+    D2 = [[INF] * (m + 1) for _ in range(n + 1)]
+    M[0][0] = 0
+    for i in range(n + 1):
+        for j in range(m + 1):
+            if i == 0 and j == 0:
+                continue
+            if j > 0:
+                I1[i][j] = min(M[i][j - 1] + o1 + e1, I1[i][j - 1] + e1)
+                I2[i][j] = min(M[i][j - 1] + o2 + e2, I2[i][j - 1] + e2)
+            if i > 0:
+                D1[i][j] = min(M[i - 1][j] + o1 + e1, D1[i - 1][j] + e1)
+                D2[i][j] = min(M[i - 1][j] + o2 + e2, D2[i - 1][j] + e2)
+            sub = M[i - 1][j - 1] + (match if p[i - 1] == t[j - 1] else x) if i > 0 and j > 0 else INF
+            M[i][j] = min(sub, I1[i][j], I2[i][j], D1[i][j], D2[i][j])
+    return M[n][m]
+
+
+def cigar_score2p(cigar, match, x, o1, e1, o2, e2):
+    """cigar_score_gap_affine2p (alignment/cigar.c:304-342) over run-length text."""
+    score, num = 0, 0
+    for ch in cigar:
+        if ch.isdigit():
+            num = num * 10 + int(ch)
+            continue
+        if ch == "M":
+            score -= match * num
+        elif ch == "X":
+            score -= x * num
+        else:
+            score -= min(o1 + e1 * num, o2 + e2 * num)
+        num = 0
+    return score
+
+
+def _long_indel_pairs(seed, n_each=12):
+    """Pairs with gap runs of 20-200 bases, where the second gap model is the cheaper one."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in (300, 800, 2000):
+        for _ in range(n_each):
+            p = "".join(rng.choice(list("ACGT"), n))
+            t, i = [], 0
+            while i < n:
+                r = rng.random()
+                if r < 0.01:
+                    t.append(str(rng.choice(list("ACGT")))); i += 1
+                elif r < 0.013:
+                    i += int(rng.integers(20, 200))
+                elif r < 0.016:
+                    t.extend(rng.choice(list("ACGT"), int(rng.integers(20, 200))))
+                else:
+                    t.append(p[i]); i += 1
+            out.append((p, "".join(t)))
+    return out
+
+
+def test_oracle_affine2p_matches_golden(oracle, utest_pairs):
+    gold = golden_alg2p()
+    r = oracle.align(utest_pairs, AFFINE2P)
+    assert set(r.status.tolist()) == {0}
+    for i, (s, c) in enumerate(gold):
+        assert (int(r.alg_score[i]), r.cigar(i)) == (s, c), "pair %d" % i
+    gold = golden_alg2p(score_only=True)
+    r = oracle.align(utest_pairs, AFFINE2P, score_only=True)
+    assert [int(s) for s in r.alg_score] == [g[0] for g in gold]
+
+
+@pytest.mark.parametrize("pen", AFFINE2P_SETS)
+def test_oracle_affine2p_vs_ref(oracle, wfa2ref, pen):
+    pairs = _random_pairs(21, 150, 400) + _long_indel_pairs(22, 6)
+    for score_only in (False, True):
+        r = oracle.align(pairs, pen, score_only=score_only)
+        w = wfa2ref.align(pairs, pen, score_only=score_only)
+        assert np.array_equal(r.status, w.status)
+        assert np.array_equal(r.alg_score, w.alg_score)
+        if not score_only:
+            for i in range(len(pairs)):
+                assert r.cigar(i) == w.cigar(i), "pair %d" % i
+
+
+def test_oracle_affine2p_vs_ref_max_steps(oracle, wfa2ref):
+    pairs = _random_pairs(23, 120, 600)
+    for cap in (5, 50, 200):
+        r = oracle.align(pairs, AFFINE2P, max_steps=cap)
+        w = wfa2ref.align(pairs, AFFINE2P, max_steps=cap)
+        assert np.array_equal(r.status, w.status)
+        ok = w.status == 0
+        assert np.array_equal(r.alg_score[ok], w.alg_score[ok])
+
+
+@pytest.mark.parametrize("pen", [AFFINE2P, (0, 4, 8, 2, 4, 3), (-1, 1, 2, 1, 5, 1)])
+def test_oracle_affine2p_optimal_vs_gotoh_and_valid(oracle, pen):
+    """Small pairs: the score is the two-piece Gotoh optimum; the CIGAR spells the
+    pair and rescores under cigar_score_gap_affine2p."""
+    pairs = _random_pairs(24, 90, 40, p_unrelated=0.3)
+    rng = np.random.default_rng(25)
+    for _ in range(20):   # a long gap inside a short pair makes the second model win
+        p = "".join(rng.choice(list("ACGT"), 40))
+        j = int(rng.integers(0, 20))
+        pairs.append((p, p[:j] + p[j + int(rng.integers(15, 20)):]))
+    r = oracle.align(pairs, pen)
+    for i, (p, t) in enumerate(pairs):
+        cig = r.cigar(i)
+        if not p and not t:
+            continue
+        assert cigar_spells(p, t, cig), (p, t, cig)
+        assert cigar_score2p(cig, *pen) == r.alg_score[i]
+        assert r.alg_score[i] == -gotoh2p_cost(p, t, *pen), (p, t)
+
+
+def test_planted_affine2p_mistake_is_caught(utest_pairs, tmp_path):
+    """M computed without its I2 input (a plausible slip in the five-component
+    recurrence) must fail the 2p golden comparison."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    src = open(os.path.join(ROOT, "oracle", "biwfa_oracle.c")).read()
+    planted = src.replace("const int32_t ins = ins1 >= ins2 ? ins1 : ins2;", "const int32_t ins = ins1;")
+    assert planted != src
+    cfile = tmp_path / "planted2p.c"
+    cfile.write_text(planted)
+    lib = tmp_path / "libplanted2p.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-I", os.path.join(ROOT, "oracle"),
+                    "-o", str(lib), str(cfile)], check=True)
+    from oracle.oracle import Oracle
+    r = Oracle(path=str(lib)).align(utest_pairs, AFFINE2P)
+    gold = golden_alg2p()
+    assert sum(1 for i, (s, c) in enumerate(gold) if r.cigar(i) != c or int(r.alg_score[i]) != s) > 0
