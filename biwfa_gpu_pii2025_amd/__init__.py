@@ -51,11 +51,22 @@ class _Pen(ctypes.Structure):
                 ("gap_opening", ctypes.c_int), ("gap_extension", ctypes.c_int)]
 
 
+class _Pen2p(ctypes.Structure):
+    _fields_ = [("match", ctypes.c_int), ("mismatch", ctypes.c_int),
+                ("gap_opening1", ctypes.c_int), ("gap_extension1", ctypes.c_int),
+                ("gap_opening2", ctypes.c_int), ("gap_extension2", ctypes.c_int)]
+
+
+GAP_AFFINE = 0
+GAP_AFFINE_2P = 1
+
+
 class _Attr(ctypes.Structure):
     _fields_ = [("penalties", _Pen), ("scope", ctypes.c_int),
                 ("max_alignment_steps", ctypes.c_int), ("device", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
-                ("queue_chunk", ctypes.c_int)]
+                ("queue_chunk", ctypes.c_int), ("distance_metric", ctypes.c_int),
+                ("affine2p_penalties", _Pen2p)]
 
 
 class _Batch(ctypes.Structure):
@@ -117,6 +128,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
         L.bw_aligner_new.restype = ctypes.c_void_p
         L.bw_aligner_new.argtypes = [ctypes.POINTER(_Attr)]
+        L.bw_attr_default.argtypes = [ctypes.POINTER(_Attr)]
         L.bw_aligner_delete.argtypes = [ctypes.c_void_p]
         L.bw_align.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.bw_align_packed2bits.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
@@ -450,12 +462,22 @@ class BatchResult:
 
 
 class WavefrontAligner:
-    """A gfx950 BiWFA aligner (the wavefront_aligner_t of WFA2-lib, wavefront/wfa.h:90-118)."""
+    """A gfx950 BiWFA aligner (the wavefront_aligner_t of WFA2-lib, wavefront/wfa.h:90-118).
+
+    penalties: (match, x, o, e) for gap-affine, or (match, x, o1, e1, o2, e2) for
+    gap-affine-2p (WFA2's distance_metric gap_affine_2p)."""
 
     def __init__(self, penalties=(0, 4, 6, 2), score_only=False, max_steps=0, device=0,
                  schedule=SCHEDULE_LENGTH_SORTED, waves_per_cu=0, queue_chunk=0):
         a = _Attr()
-        a.penalties = _Pen(*[int(v) for v in penalties])
+        lib().bw_attr_default(ctypes.byref(a))
+        if len(penalties) == 6:
+            a.distance_metric = GAP_AFFINE_2P
+            a.affine2p_penalties = _Pen2p(*[int(v) for v in penalties])
+        elif len(penalties) == 4:
+            a.penalties = _Pen(*[int(v) for v in penalties])
+        else:
+            raise ValueError("penalties: (match, x, o, e) or (match, x, o1, e1, o2, e2)")
         a.scope = SCOPE_SCORE if score_only else SCOPE_ALIGNMENT
         a.max_alignment_steps = int(max_steps)
         a.device = int(device)

@@ -60,7 +60,7 @@ def build(verbose=False, jobs=None):
         obj = os.path.join(OBJ, "biwfa_kernel_%02d.o" % part)
         steps.append(([hipcc] + HIP_FLAGS + ["-DBW_PART=%d" % part, "-c", os.path.join(CSRC, "biwfa_kernel.hip"),
                                             "-o", obj], obj))
-    for src in ("runtime.cpp", "costmodel.hip"):
+    for src in ("runtime.cpp", "costmodel.hip", "biwfa_affine2p.hip"):
         obj = os.path.join(OBJ, src.split(".")[0] + ".o")
         steps.append(([hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
     for src in ("dataset.cpp", "seqio.cpp"):

@@ -130,3 +130,50 @@ struct KParams {
 };
 
 }  // namespace bw
+
+namespace bw {
+
+// ---------------------------------------------------------------------------
+// Gap-affine-2p kernel (biwfa_affine2p.hip): one wave per pair, five
+// components (M, I1, D1, I2, D2), rings and base-case arena in a per-wave
+// global slab, ring headers in LDS.
+constexpr int kA2pMaxScope = 64;        // max(o1+e1, o2+e2, x) + 1 the header table holds
+constexpr int kA2pWavesPerBlock = 4;
+
+struct A2pParams {
+  // input batch (device)
+  const uint64_t* words;
+  const int64_t* p_woff;
+  const int32_t* p_len;
+  const int64_t* t_woff;
+  const int32_t* t_len;
+  const int32_t* order;          // pair ids in schedule order
+  int32_t n;
+  int32_t* queue;                // next order index (atomic; zeroed by biwfa2p_reset)
+  // penalties: internal (after the match<0 transform) and user (for the .alg score)
+  int32_t x, o1, e1, o2, e2, scope;
+  int32_t um, ux, uo1, ue1, uo2, ue2;
+  int32_t score_only;
+  int32_t max_steps;
+  // outputs (device, indexed by pair id)
+  int32_t* score;
+  int32_t* wf_score;
+  int32_t* status;
+  uint32_t* runs;
+  const int64_t* runs_off;       // [N+1] run capacity prefix
+  int32_t* nruns;
+  // workspace: one slab of `stride` ints per wave
+  int32_t* gws;
+  int64_t stride;
+  int64_t aligners;              // waves with a slab
+  int32_t wcap;                  // ring width (power of two)
+  int32_t arena_cap;             // ints at the slab base for the rings / base-case arena
+  int32_t bhdr;                  // base-case headers: hcap x 5 x {lo, hi, base}
+  int32_t hcap;
+  int32_t runs_tmp;              // reversed runs of one backtrace
+  int32_t rcap;
+  int32_t stack;                 // recursion stack: scap x kSubInts
+  int32_t scap;
+};
+
+}  // namespace bw

@@ -31,6 +31,10 @@ __global__ void biwfa_reset(int32_t* counters, const int32_t* init, int n);
 __global__ void biwfa_compact(const uint32_t* runs, const int64_t* runs_off, const int32_t* nruns,
                               const int64_t* out_off, uint32_t* out, int n);
 }  // namespace bw
+namespace bw2p {
+__global__ void biwfa2p_persistent(bw::A2pParams p);
+__global__ void biwfa2p_reset(int32_t* qctl, int n);
+}  // namespace bw2p
 
 namespace {
 
@@ -130,6 +134,41 @@ int internal_penalties(const bw_affine_penalties_t& p, Penalties* out) {
   return 0;
 }
 
+// Gap-affine-2p penalties, internal (after the match<0 transform) and user.
+struct Pen2p {
+  int x, o1, e1, o2, e2, scope;
+  int um, ux, uo1, ue1, uo2, ue2;
+};
+
+int internal_penalties_2p(const bw_affine2p_penalties_t& p, Pen2p* out) {
+  // wavefront_penalties_set_affine2p (wavefront_penalties.c:132-173)
+  if (p.match > 0 || p.mismatch <= 0 || p.gap_opening1 < 0 || p.gap_extension1 <= 0 || p.gap_opening2 < 0 ||
+      p.gap_extension2 <= 0) {
+    set_error("gap-affine-2p penalties must satisfy match<=0, mismatch>0, gap_opening1/2>=0, gap_extension1/2>0");
+    return BW_STATUS_INVALID_ARG;
+  }
+  Pen2p q;
+  q.um = p.match; q.ux = p.mismatch;
+  q.uo1 = p.gap_opening1; q.ue1 = p.gap_extension1; q.uo2 = p.gap_opening2; q.ue2 = p.gap_extension2;
+  if (p.match < 0) {
+    q.x = 2 * p.mismatch - 2 * p.match;
+    q.o1 = 2 * p.gap_opening1;
+    q.e1 = 2 * p.gap_extension1 - p.match;
+    q.o2 = 2 * p.gap_opening2;
+    q.e2 = 2 * p.gap_extension2 - p.match;
+  } else {
+    q.x = p.mismatch; q.o1 = p.gap_opening1; q.e1 = p.gap_extension1; q.o2 = p.gap_opening2; q.e2 = p.gap_extension2;
+  }
+  // wavefront_components_dimensions_affine2p (wavefront_components.c:101-124)
+  q.scope = std::max(std::max(q.o1 + q.e1, q.o2 + q.e2), q.x) + 1;
+  if (q.scope > bw::kA2pMaxScope) {
+    set_error("max(o1+e1, o2+e2, x)+1 exceeds the supported ring depth (64)");
+    return BW_STATUS_INVALID_ARG;
+  }
+  *out = q;
+  return 0;
+}
+
 }  // namespace
 
 // The other host sources (sequence files) report through the same text.
@@ -201,6 +240,11 @@ struct bw_dbatch {
   uint64_t owner = 0;
   // graph mode: the owner's launch sequence, captured once and replayed
   hipGraphExec_t gexec = nullptr;
+  // gap-affine-2p batch: one launch of the single-wave 2p kernel over `d_order`
+  bool a2p = false;
+  int32_t* d_order = nullptr;
+  bw::A2pParams p2{};
+  int blocks2p = 0;
 };
 
 constexpr int kMaxTiers = 8;
@@ -215,6 +259,8 @@ struct bw_aligner {
   uint64_t id = 0;                 // unique over the process (device-batch ownership)
   bw_attr_t attr{};
   Penalties pen{};
+  bool a2p = false;                // gap-affine-2p aligner (its own kernel; no tiers)
+  Pen2p p2{};
   int dev = 0, cu_count = 0;
   size_t lds_per_block_max = 0;
   hipStream_t stream = nullptr;
@@ -308,6 +354,9 @@ void bw_attr_default(bw_attr_t* attr) {
   attr->schedule = BW_SCHEDULE_LENGTH_SORTED;
   attr->waves_per_cu = 0;
   attr->queue_chunk = 0;
+  attr->distance_metric = BW_GAP_AFFINE;
+  // align_benchmark's gap-affine-2p defaults (tools/align_benchmark/align_benchmark_params.c:62-69)
+  attr->affine2p_penalties = bw_affine2p_penalties_t{0, 4, 6, 2, 24, 1};
 }
 
 bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
@@ -315,6 +364,13 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
   if (attr_in) attr = *attr_in; else bw_attr_default(&attr);
   Penalties pen;
   if (internal_penalties(attr.penalties, &pen) != 0) return nullptr;
+  Pen2p p2{};
+  if (attr.distance_metric != BW_GAP_AFFINE && attr.distance_metric != BW_GAP_AFFINE_2P) {
+    set_error("unknown distance metric");
+    return nullptr;
+  }
+  if (attr.distance_metric == BW_GAP_AFFINE_2P && internal_penalties_2p(attr.affine2p_penalties, &p2) != 0)
+    return nullptr;
   if (attr.schedule < BW_SCHEDULE_INPUT_ORDER || attr.schedule > BW_SCHEDULE_LEARNED) {
     set_error("unknown schedule");
     return nullptr;
@@ -341,6 +397,8 @@ bw_aligner_t* bw_aligner_new(const bw_attr_t* attr_in) {
   a->id = g_next_aligner_id.fetch_add(1);
   a->attr = attr;
   a->pen = pen;
+  a->a2p = attr.distance_metric == BW_GAP_AFFINE_2P;
+  a->p2 = p2;
   a->dev = attr.device;
   a->cu_count = prop.multiProcessorCount;
   // workspace slabs of the global tiers: at most a third of the device memory, 64 GiB
@@ -390,6 +448,7 @@ void bw_dbatch_free(bw_dbatch_t* b) {
   dfree(b->d_lists); dfree(b->d_init); dfree(b->d_score); dfree(b->d_wf); dfree(b->d_status);
   dfree(b->d_nruns); dfree(b->d_runs); dfree(b->d_runs_off); dfree(b->d_gws);
   dfree(b->d_items); dfree(b->d_pend); dfree(b->d_pfail); dfree(b->d_pflag);
+  dfree(b->d_order);
   delete b;
 }
 
@@ -565,6 +624,109 @@ static int initial_tier(const bw_dbatch_t* b, int pair, double width_guess) {
   return (int)T.size() - 1;
 }
 
+// Gap-affine-2p batch: inputs, outputs and one workspace slab per wave of the
+// single-wave 2p kernel.  A slab holds the two rings (2 x scope x 5 x wcap
+// offsets, wcap a power of two >= p + t + 1 of the longest pair) or, in the
+// same space, a base case's arena, then the base-case headers, the backtrace
+// run buffer and the recursion stack.  The base-case capacities come from the
+// largest score a base case can reach: 250 below a breakpoint, 500 on the
+// end-reached fallback, the max(p, t) <= 100 whole-pair case (100 mismatches
+// plus a 100-base gap), with slack.
+static int upload_2p(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t* b, const std::vector<int64_t>& runs_off,
+                     int64_t budget, bw_dbatch_t** out) {
+  const Pen2p& q = a->p2;
+  const int64_t n = b->n;
+  auto fail = [&](hipError_t e) {
+    set_error(std::string("device allocation/copy failed: ") + hipGetErrorString(e));
+    bw_dbatch_free(b);
+    return BW_STATUS_HIP_ERROR;
+  };
+  b->a2p = true;
+  const int wcap = next_pow2(std::max(b->max_pt + 1, 64));
+  const int smax = std::max(bw::kRecoveryMinScore, 100 * q.x + std::min(q.o1 + 100 * q.e1, q.o2 + 100 * q.e2)) +
+                   2 * q.scope + 64;
+  const int emin = std::max(1, std::min(q.e1, q.e2));
+  const int64_t width = std::min<int64_t>(2 * (int64_t)smax / emin + 3, (int64_t)b->max_pt + 3);
+  const int64_t ring_ints = (int64_t)2 * q.scope * 5 * wcap;
+  const int64_t arena = std::max<int64_t>(ring_ints, (int64_t)5 * (smax + 1) * width);
+  if (arena > ((int64_t)1 << 30)) {
+    set_error("gap-affine-2p workspace of one pair exceeds 4 GiB");
+    bw_dbatch_free(b);
+    return BW_STATUS_INVALID_ARG;
+  }
+  bw::A2pParams& p = b->p2;
+  p = bw::A2pParams{};
+  p.wcap = wcap;
+  p.arena_cap = (int)arena;
+  int64_t off = (arena + 63) & ~(int64_t)63;
+  p.bhdr = (int)off;
+  p.hcap = smax + 2;
+  off += (int64_t)p.hcap * 5 * 3;
+  p.runs_tmp = (int)off;
+  p.rcap = 2 * smax + 16;
+  off += p.rcap;
+  p.stack = (int)off;
+  p.scap = 64;
+  off += (int64_t)p.scap * bw::kSubInts;
+  p.stride = (off + 63) & ~(int64_t)63;
+  const int64_t slab = p.stride * 4;
+  const int64_t fit = std::min<int64_t>(budget, (int64_t)16 << 30) / slab;
+  if (fit < 1) {
+    set_error("gap-affine-2p workspace of one pair exceeds the workspace budget");
+    bw_dbatch_free(b);
+    return BW_STATUS_DEVICE_CAPACITY;
+  }
+  p.aligners = std::max<int64_t>(1, std::min<int64_t>({std::max<int64_t>(n, 1), (int64_t)a->cu_count * 8, fit}));
+  b->blocks2p = (int)((p.aligners + bw::kA2pWavesPerBlock - 1) / bw::kA2pWavesPerBlock);
+  hipError_t e;
+  if ((e = dalloc(&b->d_words, in->num_words)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_pwoff, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_plen, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_twoff, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_tlen, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_order, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_score, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_wf, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_status, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_nruns, n)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_runs, b->runs_cap)) != hipSuccess) return fail(e);
+  if ((e = dalloc(&b->d_runs_off, n + 1)) != hipSuccess) return fail(e);
+  b->gws_bytes = p.aligners * slab;
+  if ((e = hipMalloc(reinterpret_cast<void**>(&b->d_gws), b->gws_bytes)) != hipSuccess) return fail(e);
+  hipStream_t s = a->stream;
+  if ((e = hipMemcpyAsync(b->d_words, in->words, sizeof(uint64_t) * in->num_words, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+  if (n > 0) {
+    if ((e = hipMemcpyAsync(b->d_pwoff, in->pattern_word_offset, sizeof(int64_t) * n, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(b->d_plen, in->pattern_length, sizeof(int32_t) * n, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(b->d_twoff, in->text_word_offset, sizeof(int64_t) * n, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(b->d_tlen, in->text_length, sizeof(int32_t) * n, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+  }
+  if ((e = hipMemcpyAsync(b->d_runs_off, runs_off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e);
+  p.words = b->d_words;
+  p.p_woff = b->d_pwoff; p.p_len = b->d_plen;
+  p.t_woff = b->d_twoff; p.t_len = b->d_tlen;
+  p.order = b->d_order;
+  p.n = (int32_t)n;
+  p.queue = a->d_counters;
+  p.x = q.x; p.o1 = q.o1; p.e1 = q.e1; p.o2 = q.o2; p.e2 = q.e2; p.scope = q.scope;
+  p.um = q.um; p.ux = q.ux; p.uo1 = q.uo1; p.ue1 = q.ue1; p.uo2 = q.uo2; p.ue2 = q.ue2;
+  p.score_only = a->attr.scope == BW_SCOPE_SCORE;
+  p.max_steps = a->attr.max_alignment_steps > 0 ? a->attr.max_alignment_steps : INT32_MAX;
+  p.score = b->d_score; p.wf_score = b->d_wf; p.status = b->d_status;
+  p.runs = b->d_runs; p.runs_off = b->d_runs_off; p.nruns = b->d_nruns;
+  p.gws = b->d_gws;
+  *out = b;
+  const int sched = a->schedule == BW_SCHEDULE_LEARNED ? BW_SCHEDULE_LENGTH_SORTED : a->schedule;
+  const int st = bw_set_schedule(a, b, sched, nullptr);
+  if (st != 0) {
+    bw_dbatch_free(b);
+    *out = nullptr;
+    return st;
+  }
+  return 0;
+}
+
 extern "C" {
 
 static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out, int64_t budget = 0) {
@@ -603,6 +765,7 @@ static int upload_impl(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t** out,
   }
   b->runs_cap = runs_off[n];
   b->owner = a->id;
+  if (a->a2p) return upload_2p(a, in, b, runs_off, budget > 0 ? std::min(budget, a->ws_budget) : a->ws_budget, out);
   b->tiers = plan_tiers(a, b, budget > 0 ? std::min(budget, a->ws_budget) : a->ws_budget);
   if (const int st = check_lds_layout(a, b->tiers[0])) {
     bw_dbatch_free(b);
@@ -688,6 +851,23 @@ int bw_set_schedule(bw_aligner_t* a, bw_dbatch_t* b, int schedule, const float* 
     return BW_STATUS_INVALID_ARG;
   }
   const int64_t n = b->n;
+  if (b->a2p) {   // one list, input order or longest (p + t) first
+    if (schedule == BW_SCHEDULE_LEARNED) {
+      set_error("the learned schedule is for gap-affine batches");
+      return BW_STATUS_INVALID_ARG;
+    }
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    if (schedule == BW_SCHEDULE_LENGTH_SORTED)
+      std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return (int64_t)b->h_plen[x] + b->h_tlen[x] > (int64_t)b->h_plen[y] + b->h_tlen[y];
+      });
+    a->schedule = schedule;
+    HIP_TRY(hipSetDevice(a->dev));
+    if (n > 0) HIP_TRY(hipMemcpyAsync(b->d_order, order.data(), 4 * n, hipMemcpyHostToDevice, a->stream));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    return 0;
+  }
   const Penalties& q = a->pen;
   // cost key (larger first) and initial tier of every pair
   std::vector<double> key(n);
@@ -838,12 +1018,45 @@ static void record_shapes(bw_aligner_t* a, const bw_dbatch_t* b) {
   for (int t = 0; t < (int)b->tiers.size() && t < kMaxTiers; ++t) record_shape(a, t, b->tiers[t]);
 }
 
+// Gap-affine-2p: a counter reset and one launch of the single-wave 2p kernel
+// (eager launches also in graph mode).
+static int launch_2p(bw_aligner_t* a, bw_dbatch_t* b) {
+  hipStream_t s = a->stream;
+  a->last_counters = a->d_counters;
+  bw_tier_shape_t& sh = a->tier_shape[0];
+  sh = bw_tier_shape_t{};
+  sh.lds = 0;
+  sh.waves_per_aligner = 1;
+  sh.ring_width = b->p2.wcap;
+  sh.blocks = b->blocks2p;
+  sh.waves_per_block = bw::kA2pWavesPerBlock;
+  sh.aligners = b->p2.aligners;
+  std::snprintf(sh.kernel, sizeof(sh.kernel), "%s", "bw2p::biwfa2p_persistent(bw::A2pParams)");
+  HIP_TRY(hipEventRecord(a->ev[0], s));
+  HIP_TRY(hipEventRecord(a->ev[1], s));
+  HIP_TRY(hipEventRecord(a->tev[0], s));
+  hipLaunchKernelGGL(bw2p::biwfa2p_reset, dim3(1), dim3(64), 0, s, a->d_counters, (int)b->n);
+  HIP_TRY(hipGetLastError());
+  bw::A2pParams p = b->p2;
+  p.queue = a->d_counters;
+  void* args[] = {&p};
+  HIP_TRY(hipLaunchKernel(reinterpret_cast<const void*>(&bw2p::biwfa2p_persistent), dim3(b->blocks2p),
+                          dim3(bw::kWave * bw::kA2pWavesPerBlock), args, 0, s));
+  HIP_TRY(hipEventRecord(a->tev[1], s));
+  HIP_TRY(hipEventRecord(a->ev[2], s));
+  HIP_TRY(hipEventRecord(a->ev[3], s));
+  a->last_ntiers = 1;
+  a->last_graph = false;
+  return 0;
+}
+
 int bw_dbatch_align(bw_aligner_t* a, bw_dbatch_t* b) {
   if (!a || !b) return BW_STATUS_INVALID_ARG;
   if (const int st = check_owner(a, b)) return st;
   HIP_TRY(hipSetDevice(a->dev));
   if (b->n == 0) return 0;
   hipStream_t s = a->stream;
+  if (b->a2p) return launch_2p(a, b);
   const int nt = (int)b->tiers.size();
   a->last_counters = a->d_counters;
   if (a->graph) {
@@ -1080,6 +1293,10 @@ static int min_team_for(int wcap) {
 int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, int repeats,
                 bw_tuning_t* best) {
   if (!a || !sample) return BW_STATUS_INVALID_ARG;
+  if (a->a2p) {
+    set_error("autotuning tunes the gap-affine tiers; a gap-affine-2p aligner has none");
+    return BW_STATUS_INVALID_ARG;
+  }
   drop_one(a);   // the single-pair batch was planned for the tuning in force
   const int sched0 = a->schedule;
   if (repeats < 1) repeats = 1;
@@ -1381,6 +1598,10 @@ int bw_test_set_ready_guard(bw_aligner_t* a, int64_t ready_ticks, int withhold_r
 int bw_align_device_noalloc(bw_aligner_t* a, const bw_batch_t* in, const bw_device_result_t* out) {
   if (!a || !in || !out) {
     set_error("bw_align_device_noalloc: null aligner, batch or result");
+    return BW_STATUS_INVALID_ARG;
+  }
+  if (a->a2p) {
+    set_error("bw_align_device_noalloc aligns gap-affine pairs; this aligner is gap-affine-2p");
     return BW_STATUS_INVALID_ARG;
   }
   if (in->num_pairs < 0 || in->num_pairs > kNaMaxPairs) {

@@ -4,7 +4,8 @@
  * the function says otherwise.
  *
  * Results are bit-exact with WFA2-lib v2.3 `wavefront_align` configured for
- * exact BiWFA: gap_affine penalties, memory_mode = ultralow, heuristic = none
+ * exact BiWFA: gap_affine (or gap_affine_2p) penalties, memory_mode = ultralow,
+ * heuristic = none
  * (score from the CIGAR as align_benchmark writes it, CIGAR operations byte for
  * byte).  Paths below are relative to the reference checkout.
  *
@@ -54,6 +55,25 @@ typedef struct {
   int gap_extension;  /* > 0  */
 } bw_affine_penalties_t;                 /* alignment/affine_penalties.h */
 
+typedef struct {
+  int match;           /* <= 0 (match < 0 uses WFA2's Eizenga transform) */
+  int mismatch;        /* > 0  */
+  int gap_opening1;    /* >= 0 */
+  int gap_extension1;  /* > 0  */
+  int gap_opening2;    /* >= 0 */
+  int gap_extension2;  /* > 0  */
+} bw_affine2p_penalties_t;               /* alignment/affine2p_penalties.h */
+
+/* Distance metric (wavefront/wavefront_penalties.h:41-48 distance_metric_t: gap_affine,
+ * gap_affine_2p).  Gap-affine-2p scores a gap run of length n as
+ * min(o1 + e1*n, o2 + e2*n) and runs on its own single-wave kernel; the tiers,
+ * the learned schedule, autotuning and the no-allocation entry point are for
+ * gap-affine and return BW_STATUS_INVALID_ARG for a gap-affine-2p aligner. */
+typedef enum {
+  BW_GAP_AFFINE = 0,
+  BW_GAP_AFFINE_2P = 1
+} bw_distance_metric_t;
+
 typedef enum {
   BW_SCOPE_ALIGNMENT = 0,                /* score + CIGAR (compute_alignment) */
   BW_SCOPE_SCORE = 1                     /* score only    (compute_score)     */
@@ -73,7 +93,9 @@ typedef struct {
   int schedule;                          /* bw_schedule_t                              */
   int waves_per_cu;                      /* 0: autotuned                               */
   int queue_chunk;                       /* 0: autotuned                               */
-} bw_attr_t;
+  int distance_metric;                   /* bw_distance_metric_t (default BW_GAP_AFFINE) */
+  bw_affine2p_penalties_t affine2p_penalties;   /* BW_GAP_AFFINE_2P: default {0,4,6,2,24,1} */
+} bw_attr_t;                             /* wavefront_attributes.h wavefront_aligner_attr_t */
 
 typedef struct bw_aligner bw_aligner_t;
 

@@ -141,3 +141,63 @@ def test_integration_doc_shows_the_stub():
     src = open(os.path.join(ROOT, "tests", "c_stub", "alg_line.c")).read()
     body = src[src.index("int main"):]
     assert body.strip() in doc
+
+
+def test_attr_default_and_affine2p_penalty_checks():
+    """bw_attr_default fills gap-affine with WFA2's defaults and the 2p penalties with
+    align_benchmark's (0,4,6,2,24,1); a gap-affine-2p aligner with bad penalties or
+    too deep a ring is refused before any device is touched."""
+    L = bw.lib()
+    a = bw._Attr()
+    L.bw_attr_default(ctypes.byref(a))
+    assert (a.penalties.match, a.penalties.mismatch, a.penalties.gap_opening, a.penalties.gap_extension) == \
+        (0, 4, 6, 2)
+    assert a.distance_metric == bw.GAP_AFFINE
+    p2 = a.affine2p_penalties
+    assert (p2.match, p2.mismatch, p2.gap_opening1, p2.gap_extension1, p2.gap_opening2, p2.gap_extension2) == \
+        (0, 4, 6, 2, 24, 1)
+    a.distance_metric = bw.GAP_AFFINE_2P
+    a.affine2p_penalties = bw._Pen2p(0, 4, 6, 0, 24, 1)     # e1 must be > 0
+    assert not L.bw_aligner_new(ctypes.byref(a))
+    assert "2p" in bw.last_error()
+    a.affine2p_penalties = bw._Pen2p(0, 4, 6, 2, 80, 1)     # ring deeper than 64 scores
+    assert not L.bw_aligner_new(ctypes.byref(a))
+    assert "64" in bw.last_error()
+    a.distance_metric = 7
+    assert not L.bw_aligner_new(ctypes.byref(a))
+    assert "metric" in bw.last_error()
+
+
+def test_ctypes_mirror_matches_the_header(tmp_path):
+    """The ctypes structures of the Python front-end have the sizes and field offsets
+    the C compiler gives the header's structs."""
+    import subprocess
+    structs = {"bw_attr_t": (bw._Attr, ["penalties", "scope", "max_alignment_steps", "device", "schedule",
+                                         "waves_per_cu", "queue_chunk", "distance_metric", "affine2p_penalties"]),
+               "bw_batch_t": (bw._Batch, ["num_pairs", "num_words", "words", "pattern_word_offset",
+                                           "pattern_length", "text_word_offset", "text_length"]),
+               "bw_result_t": (bw._Result, ["score", "wf_score", "status", "cigar_offset", "cigar_runs",
+                                             "cigar_capacity", "cigar_total"]),
+               "bw_tuning_t": (bw._Tuning, ["lds_ring_width", "lds_seq_cap", "team_waves", "queue_chunk"]),
+               "bw_tier_shape_t": (bw._TierShape, ["lds", "waves_per_aligner", "ring_width", "blocks",
+                                                    "waves_per_block", "aligners", "kernel"]),
+               "bw_device_result_t": (bw._DevResult, ["score", "wf_score", "status", "num_runs", "runs",
+                                                       "run_offset"])}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", "#include \"biwfa_mi355x.h\"", "int main(void) {"]
+    for name, (_, fields) in structs.items():
+        lines.append('  printf("%s %%zu\\n", sizeof(%s));' % (name, name))
+        for f in fields:
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (name, f, name, f))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c99", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for name, (cls, fields) in structs.items():
+        assert int(got[name]) == ctypes.sizeof(cls), name
+        for f in fields:
+            assert int(got[name + "." + f]) == getattr(cls, f).offset, name + "." + f
