@@ -9,12 +9,13 @@ schedule, with the plain length-sorted schedule timed on the same batch in the
 same run (fewer steps, stated in the line) and both outputs compared.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload C4]
+    python bench.py --workload C1 --affine2p      # gap-affine-2p (0,4,6,2,24,1) on the 2p kernel
 
 Prints one JSON line.  Besides the contract fields it carries
   schedules     -- plain vs learned: ms/step of each, learned_vs_plain, identical_results,
                    cost-model training and prediction figures,
-  roofline      -- algorithmic int32 work (11 ops per wavefront cell + 4 per 64-bit word
-                   compared, counted by the CPU restatement) per step over the total
+  roofline      -- algorithmic int32 work (11 ops per wavefront cell, 16 under gap-affine-2p,
+                   + 4 per 64-bit word compared, counted by the CPU restatement) per step over the total
                    kernel time, against the int32 VALU peak; names the dominant kernel,
   hbm           -- compulsory HBM bytes and the dominant kernel's measured HBM traffic
                    (committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes) per kernel second,
@@ -35,6 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0                  # MI355X HBM3E
+AFFINE2P = (0, 4, 6, 2, 24, 1)         # WFA2's gap-affine-2p utest penalties
 CLOCK_GHZ = 2.4                        # MI355X peak engine clock
 # int32 VALU peak: 256 CUs x 64 lanes per clock
 INT32_PEAK_TOPS = 256 * 64 * CLOCK_GHZ * 1e9 / 1e12
@@ -54,15 +56,17 @@ def compulsory_bytes(p_len, t_len, nruns):
     return int(np.sum(8 * words) + len(p_len) * (24 + 4 + 16) + 4 * int(np.sum(nruns)))
 
 
-def profile_json(kind, workload, n):
-    """A committed profiles/*_<kind>.json summary taken on this same workload, or None."""
+def profile_json(kind, workload, n, penalties=(0, 4, 6, 2)):
+    """A committed profiles/*_<kind>.json summary taken on this same workload (and
+    penalties), or None."""
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_%s.json" % kind))):
         try:
             with open(path) as f:
                 t = json.load(f)
         except (OSError, ValueError):
             continue
-        if t.get("workload") == workload and int(t.get("pairs", -1)) == n:
+        if t.get("workload") == workload and int(t.get("pairs", -1)) == n and \
+                list(t.get("penalties", [0, 4, 6, 2])) == list(penalties):
             t["_path"] = os.path.relpath(path, ROOT)
             return t
     return None
@@ -107,7 +111,7 @@ def gather_runs(res, idx):
     return off, runs
 
 
-def cpu_baseline(ds, res, threads, target_wall_s=8.0, every=None):
+def cpu_baseline(ds, res, threads, target_wall_s=8.0, every=None, penalties=(0, 4, 6, 2)):
     """Times the reference BiWFA on all `threads` host cores over a deterministic
     subsample of the workload (every k-th pair, k sized for about target_wall_s of
     wall time; the sample is repeated when one pass is shorter) and checks the GPU
@@ -122,16 +126,16 @@ def cpu_baseline(ds, res, threads, target_wall_s=8.0, every=None):
     if every is None:
         # calibrate on every 1000th pair (at least 64 pairs), then size the stride
         probe = np.arange(0, n, max(1, n // max(64, n // 1000)), dtype=np.int64)
-        r = eng.align(view(probe), threads=threads)
+        r = eng.align(view(probe), penalties, threads=threads)
         per_pair_wall = max(r.elapsed / len(probe), 1e-9)
         m = int(min(n, max(64, target_wall_s / per_pair_wall)))
         every = max(1, n // m)
     idx = np.arange(0, n, every, dtype=np.int64)
-    eng.align(view(idx[:max(1, len(idx) // 100)]), threads=threads)     # 1 % warm-up
-    r = eng.align(view(idx), threads=threads)
+    eng.align(view(idx[:max(1, len(idx) // 100)]), penalties, threads=threads)     # 1 % warm-up
+    r = eng.align(view(idx), penalties, threads=threads)
     wall, reps = r.elapsed, 1
     while wall < target_wall_s * 0.5 and reps < 64:
-        wall += eng.align(view(idx), threads=threads).elapsed
+        wall += eng.align(view(idx), penalties, threads=threads).elapsed
         reps += 1
     cpu_pairs_s = len(idx) * reps / wall
     s_bad = int(np.sum(res.score[idx] != r.alg_score))
@@ -168,6 +172,8 @@ def main():
                     help="replay the captured launch sequence (bw_set_graph); per-tier times read 0")
     ap.add_argument("--train-pairs", type=int, default=None,
                     help="training pairs for the cost model (default: the full 50,000-pair set)")
+    ap.add_argument("--affine2p", action="store_true",
+                    help="gap-affine-2p penalties (0,4,6,2,24,1) on the 2p kernel, length-sorted schedule")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -190,7 +196,10 @@ def main():
     import biwfa_gpu_pii2025_amd as bw
     cfg = bw.CONFIGS[args.workload]
     n = args.pairs or cfg.num_pairs
-    schedule = args.schedule or ("learned" if args.workload == "C4" else "length")
+    schedule = args.schedule or ("learned" if args.workload == "C4" and not args.affine2p else "length")
+    if args.affine2p and schedule == "learned":
+        ap.error("the learned schedule is for gap-affine batches")
+    penalties = AFFINE2P if args.affine2p else bw.configs.PENALTIES
     compare = schedule == "learned"
     t0 = time.time()
     ds = bw.generate_dataset(cfg, num_pairs=n)
@@ -198,7 +207,7 @@ def main():
     log("generated %d pairs of %s in %.1f s" % (n, args.workload, time.time() - t0))
 
     base_sched = bw.SCHEDULE_INPUT_ORDER if schedule == "input" else bw.SCHEDULE_LENGTH_SORTED
-    aligner = bw.WavefrontAligner(penalties=bw.configs.PENALTIES, schedule=base_sched)
+    aligner = bw.WavefrontAligner(penalties=penalties, schedule=base_sched)
     if args.tuning:
         aligner.tuning = json.loads(args.tuning)
     if args.graph:
@@ -301,15 +310,16 @@ def main():
         "dtype": "int32",
         "data": "synthetic (seeded generator, seed %d)" % cfg.seed,
         "config": {
-            "workload": "%s: %s; %d pairs; gap-affine x=4 o=6 e=2; score+CIGAR; %s" % (
+            "workload": "%s: %s; %d pairs; %s; score+CIGAR; %s" % (
                 cfg.name, cfg.description, n,
+                "gap-affine-2p x=4 o1=6 e1=2 o2=24 e2=1" if args.affine2p else "gap-affine x=4 o=6 e=2",
                 "learned (cost-model) schedule; plain length-sorted timed beside it on %d step(s)" % args.plain_steps
                 if compare else "length-sorted schedule" if schedule == "length" else "input-order schedule"),
             "pairs": n,
             "schedule": schedule,
             "graph_replay": bool(args.graph),
             "pattern_length": [int(packed.p_len.min()), int(packed.p_len.max())],
-            "penalties": list(bw.configs.PENALTIES),
+            "penalties": list(penalties),
         },
         "gcups": round(float(np.sum(packed.p_len.astype(np.float64) * packed.t_len)) / (ms_per_step / 1000.0) / 1e9, 3),
         "kernel_ms": {"total": round(kern_ms, 4), "tiers": tier_ms, "tier_pairs": tier_pairs},
@@ -320,7 +330,7 @@ def main():
     }
 
     # algorithmic int32 work of the step (counted by the CPU restatement)
-    counts = profile_json("work_counts", args.workload, n)
+    counts = profile_json("work_counts", args.workload, n, penalties)
     s2_all = float(np.sum(res.score.astype(np.float64) ** 2))
     if counts is not None:
         ops = s2_all * counts["ops_per_score_sq"]
@@ -342,12 +352,13 @@ def main():
         }
         out["cells_per_s"] = round(cells / kern_s, 1)
         lds_bytes = 32.0 * cells + 16.0 * words
-        out["lds_model"] = {"achieved": round(lds_bytes / kern_s / 1e9, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(lds_bytes / kern_s / 1e9 / LDS_PEAK_GBS, 5),
-                            "model": "32 B per cell + 16 B per word"}
+        if not args.affine2p:   # the 2p kernel keeps its rings in HBM
+            out["lds_model"] = {"achieved": round(lds_bytes / kern_s / 1e9, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(lds_bytes / kern_s / 1e9 / LDS_PEAK_GBS, 5),
+                                "model": "32 B per cell + 16 B per word"}
     comp = compulsory_bytes(packed.p_len, packed.t_len, nruns)
     hbm = {"compulsory_bytes": comp, "compulsory_gbs": round(comp / kern_s / 1e9, 3), "peak": HBM_PEAK_GBS}
-    traffic = profile_json("traffic", args.workload, n)
+    traffic = profile_json("traffic", args.workload, n, penalties)
     if traffic is not None and traffic.get("kernel") == dom_shape.get("kernel"):
         tb = int(traffic["traffic_bytes"])
         dom_s = float(tier_ms[dom]) / 1000.0
@@ -365,7 +376,8 @@ def main():
         cpu = host_cpu()
         threads = cpu["usable"]
         every = 100 if (args.workload == "C4" and n == cfg.num_pairs) else None
-        base, parity, idx = cpu_baseline(ds, res, threads, target_wall_s=args.cpu_seconds, every=every)
+        base, parity, idx = cpu_baseline(ds, res, threads, target_wall_s=args.cpu_seconds, every=every,
+                                         penalties=penalties)
         base.update({"logical_cores": cpu["logical"], "physical_cores": cpu["physical"], "cpu_model": cpu["model"]})
         # the subsample's mean work against the whole batch's (work ~ score^2)
         s2 = res.score.astype(np.float64) ** 2

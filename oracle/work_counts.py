@@ -21,6 +21,11 @@ as sum_all(s^2) * sum_sample(O) / sum_sample(s^2) (a ratio estimator; its spread
 over sample halves is reported with the counts).
 
     python -m oracle.work_counts C4 --every 100 --out profiles/c4_200k_work_counts.json
+
+Under gap-affine-2p (--affine2p: WFA2's utest penalties 0,4,6,2,24,1) a cell has
+seven inputs and five outputs: O = 16 C + 4 W (I1, I2 = max + add each; D1, D2 =
+max each; max(I1, I2), max(D1, D2); M = add + 2 max; bounds = sub + 2 compares +
+2 selects; wavefront_compute_affine2p.c:45-108).
 """
 import argparse
 import json
@@ -35,7 +40,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
-def count(workload, every, threads=0, num_pairs=None):
+AFFINE2P = (0, 4, 6, 2, 24, 1)
+
+
+def count(workload, every, threads=0, num_pairs=None, affine2p=False):
     import biwfa_gpu_pii2025_amd as bw
     from oracle.oracle import Oracle
     cfg = bw.CONFIGS[workload]
@@ -44,12 +52,13 @@ def count(workload, every, threads=0, num_pairs=None):
     idx = np.arange(0, n, every, dtype=np.int64)
     view = (ds.seq, ds.p_off[idx], ds.p_len[idx], ds.t_off[idx], ds.t_len[idx])
     t0 = time.time()
-    r = Oracle().align(view, threads=threads)
+    pen = AFFINE2P if affine2p else (0, 4, 6, 2)
+    r = Oracle().align(view, pen, threads=threads)
     el = time.time() - t0
     c = np.asarray(r.cells, np.float64)
     w = np.asarray(r.words, np.float64)
     s2 = np.asarray(r.alg_score, np.float64) ** 2
-    o = 11.0 * c + 4.0 * w
+    o = (16.0 if affine2p else 11.0) * c + 4.0 * w
     # spread of the ratio estimator: the two halves of the sample against each other
     half = np.arange(len(idx)) % 2 == 0
     ra, rb = o[half].sum() / s2[half].sum(), o[~half].sum() / s2[~half].sum()
@@ -63,8 +72,10 @@ def count(workload, every, threads=0, num_pairs=None):
         "cells_per_score_sq_cv": float(np.std(c / np.maximum(s2, 1)) / np.mean(c / np.maximum(s2, 1))),
         "status_nonzero": int(np.sum(r.status != 0)),
         "oracle_seconds": round(el, 1), "threads": threads or os.cpu_count(),
-        "ops_model": "O = 11*C + 4*W int32 ops (C cells, W 64-bit words compared)",
-        "command": "python -m oracle.work_counts %s --every %d" % (workload, every),
+        "penalties": list(pen),
+        "ops_model": "O = %d*C + 4*W int32 ops (C cells, W 64-bit words compared)" % (16 if affine2p else 11),
+        "command": "python -m oracle.work_counts %s --every %d%s" % (workload, every,
+                                                                    " --affine2p" if affine2p else ""),
     }
 
 
@@ -75,8 +86,9 @@ def main():
     ap.add_argument("--pairs", type=int, default=None)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--affine2p", action="store_true")
     a = ap.parse_args()
-    res = count(a.workload, a.every, a.threads, a.pairs)
+    res = count(a.workload, a.every, a.threads, a.pairs, a.affine2p)
     text = json.dumps(res, indent=1)
     if a.out:
         with open(a.out, "w") as f:
