@@ -2237,7 +2237,7 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
 
 template <class K>
 __global__ __launch_bounds__(K::kNW > 1 ? K::kNW * kWave : kWave * kWavesPerBlock)
-__attribute__((amdgpu_waves_per_eu(K::kLds ? 4 : 3, 8)))
+__attribute__((amdgpu_waves_per_eu(K::kLds || K::kNW == 1 ? 4 : 3, 8)))
 void biwfa_persistent(KParams p) {
   extern __shared__ int32_t lds[];
   __shared__ int32_t tred[K::kNW > 1 ? 2 * K::kNW * kRedSlots : 1];

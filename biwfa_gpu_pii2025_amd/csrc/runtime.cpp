@@ -645,10 +645,15 @@ static int upload_2p(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t* b, cons
   const int wcap = next_pow2(std::max(b->max_pt + 1, 64));
   const int smax = std::max(bw::kRecoveryMinScore, 100 * q.x + std::min(q.o1 + 100 * q.e1, q.o2 + 100 * q.e2)) +
                    2 * q.scope + 64;
+  // a base case's arena: five components per score, the score-s wavefront at
+  // most 2s/e_min + 3 diagonals wide (its limits grow by one per gap extension)
+  // and never wider than p + t + 3
   const int emin = std::max(1, std::min(q.e1, q.e2));
-  const int64_t width = std::min<int64_t>(2 * (int64_t)smax / emin + 3, (int64_t)b->max_pt + 3);
+  int64_t arena_need = 0;
+  for (int sc = 0; sc <= smax; ++sc)
+    arena_need += 5 * std::min<int64_t>(2 * (int64_t)sc / emin + 3, (int64_t)b->max_pt + 3);
   const int64_t ring_ints = (int64_t)2 * q.scope * 5 * wcap;
-  const int64_t arena = std::max<int64_t>(ring_ints, (int64_t)5 * (smax + 1) * width);
+  const int64_t arena = std::max<int64_t>(ring_ints, arena_need);
   if (arena > ((int64_t)1 << 30)) {
     set_error("gap-affine-2p workspace of one pair exceeds 4 GiB");
     bw_dbatch_free(b);
@@ -670,13 +675,20 @@ static int upload_2p(bw_aligner_t* a, const bw_batch_t* in, bw_dbatch_t* b, cons
   off += (int64_t)p.scap * bw::kSubInts;
   p.stride = (off + 63) & ~(int64_t)63;
   const int64_t slab = p.stride * 4;
-  const int64_t fit = std::min<int64_t>(budget, (int64_t)16 << 30) / slab;
+  const int64_t fit = budget / slab;
   if (fit < 1) {
     set_error("gap-affine-2p workspace of one pair exceeds the workspace budget");
     bw_dbatch_free(b);
     return BW_STATUS_DEVICE_CAPACITY;
   }
-  p.aligners = std::max<int64_t>(1, std::min<int64_t>({std::max<int64_t>(n, 1), (int64_t)a->cu_count * 8, fit}));
+  // as many waves as the CUs hold at once (the kernel's occupancy), the pairs
+  // need and the workspace budget allows
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&bw2p::biwfa2p_persistent),
+                                                   bw::kWave * bw::kA2pWavesPerBlock, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 2;
+  const int64_t resident = (int64_t)a->cu_count * per_cu * bw::kA2pWavesPerBlock;
+  p.aligners = std::max<int64_t>(1, std::min<int64_t>({std::max<int64_t>(n, 1), resident, fit}));
   b->blocks2p = (int)((p.aligners + bw::kA2pWavesPerBlock - 1) / bw::kA2pWavesPerBlock);
   hipError_t e;
   if ((e = dalloc(&b->d_words, in->num_words)) != hipSuccess) return fail(e);
