@@ -131,7 +131,8 @@ def cpu_baseline(ds, res, threads, target_wall_s=8.0, every=None, penalties=(0, 
         m = int(min(n, max(64, target_wall_s / per_pair_wall)))
         every = max(1, n // m)
     idx = np.arange(0, n, every, dtype=np.int64)
-    eng.align(view(idx[:max(1, len(idx) // 100)]), penalties, threads=threads)     # 1 % warm-up
+    if len(idx) >= 100:   # 1 % warm-up (a sample of very long pairs is too dear to warm up on)
+        eng.align(view(idx[:len(idx) // 100]), penalties, threads=threads)
     r = eng.align(view(idx), penalties, threads=threads)
     wall, reps = r.elapsed, 1
     while wall < target_wall_s * 0.5 and reps < 64:
@@ -165,6 +166,8 @@ def main():
     ap.add_argument("--plain-warmup", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="wall seconds of the CPU baseline")
+    ap.add_argument("--cpu-every", type=int, default=None, metavar="K",
+                    help="CPU baseline on every K-th pair (default: sized for --cpu-seconds)")
     ap.add_argument("--autotune", type=int, default=0, metavar="N",
                     help="tune launch shapes on an N-pair sample of the workload before timing")
     ap.add_argument("--tuning", default=None, help="JSON launch tuning (bw_tuning_t fields)")
@@ -375,7 +378,7 @@ def main():
         t0 = time.time()
         cpu = host_cpu()
         threads = cpu["usable"]
-        every = 100 if (args.workload == "C4" and n == cfg.num_pairs) else None
+        every = args.cpu_every or (100 if (args.workload == "C4" and n == cfg.num_pairs) else None)
         base, parity, idx = cpu_baseline(ds, res, threads, target_wall_s=args.cpu_seconds, every=every,
                                          penalties=penalties)
         base.update({"logical_cores": cpu["logical"], "physical_cores": cpu["physical"], "cpu_model": cpu["model"]})
