@@ -608,7 +608,7 @@ DI long long bt_max(const Ctx& c, int score, int matrix, int k) {
       const int32_t* h = c.bh + (sj * NC + comp) * 3;
       const int lo = h[0], hi = h[1], base = h[2];
       const int kk = k + dk;
-      if (lo != kAbsent && lo <= kk && kk <= hi) v = ((long long)(c.wd[base + kk] + add) << 4) | type;
+      if (lo != kAbsent && lo <= kk && kk <= hi) v = (long long)(c.wd[base + kk] + add) * 16 | type;
     }
   }
   const int vlo = (int)(uint32_t)(unsigned long long)v, vhi = (int)((unsigned long long)v >> 32);

@@ -1596,7 +1596,7 @@ DI long long bt_src(const Ctx<K>& c, const Dir& b, int score, int comp, int k, i
   int present;
   const Wf w = get_wf(c, b, score, 0, comp, &present);
   if (!present || !(w.lo <= k && k <= w.hi)) return (long long)kOffsetNull;
-  return ((long long)(uni(ld_off(c, w.base + k)) + add) << 4) | type;
+  return (long long)(uni(ld_off(c, w.base + k)) + add) * 16 | type;
 }
 DI long long max64(long long a, long long b) { return a >= b ? a : b; }
 
@@ -1632,7 +1632,7 @@ DI long long bt_max_mem(const Ctx<K>& c, int score, int matrix, int k) {
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     if (!en[j]) continue;
-    const long long v = in[j] ? (((long long)(uni(off[j]) + ad[j]) << 4) | ty[j]) : (long long)kOffsetNull;
+    const long long v = in[j] ? ((long long)(uni(off[j]) + ad[j]) * 16 | ty[j]) : (long long)kOffsetNull;
     m = max64(m, v);
   }
   return m;

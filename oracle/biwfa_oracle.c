@@ -558,7 +558,7 @@ static int64_t bt_src(aligner_t* a, int score, int comp, int k, int add, int typ
   if (score < 0 || score >= a->nwfs) return OFFSET_NULL;
   const wf_t* w = &a->wfs[score].c[comp];
   if (!w->present || !(w->lo <= k && k <= w->hi)) return OFFSET_NULL;
-  return ((int64_t)(w->off[k - w->clo] + add) << 4) | type;
+  return (int64_t)(w->off[k - w->clo] + add) * 16 | type;
 }
 static inline int64_t max64(int64_t p, int64_t q) { return p >= q ? p : q; }
 
