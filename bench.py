@@ -38,10 +38,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0                  # MI355X HBM3E
 AFFINE2P = (0, 4, 6, 2, 24, 1)         # WFA2's gap-affine-2p utest penalties
 CLOCK_GHZ = 2.4                        # MI355X peak engine clock
+NUM_CUS = 256
 # int32 VALU peak: 256 CUs x 64 lanes per clock
-INT32_PEAK_TOPS = 256 * 64 * CLOCK_GHZ * 1e9 / 1e12
+INT32_PEAK_TOPS = NUM_CUS * 64 * CLOCK_GHZ * 1e9 / 1e12
 # LDS: 256 CUs x 128 bytes per clock
-LDS_PEAK_GBS = 256 * 128 * CLOCK_GHZ
+LDS_PEAK_GBS = NUM_CUS * 128 * CLOCK_GHZ
 
 
 def log(msg):
@@ -373,6 +374,21 @@ def main():
         if "roofline" in out:
             out["roofline"]["traffic"] = tb
     out["hbm"] = hbm
+    # VALU issue of the dominant kernel: the committed SQ profile's wave
+    # instructions per cell (or per pair) at this run's rate.  A wave64 VALU
+    # instruction holds one of a CU's 4 SIMDs for 4 clocks, so VALU instructions
+    # per CU per clock is the SIMDs' VALU busy fraction.
+    sq = profile_json("sq", args.workload, n, penalties)
+    if sq is not None and "roofline" in out and tier_ms:
+        if "valu_per_cell" in sq:
+            valu = sq["valu_per_cell"] * out["roofline"]["cells_per_step"]
+            salu = sq["salu_per_cell"] * out["roofline"]["cells_per_step"]
+        else:
+            valu, salu = sq["valu_per_pair"] * n, sq["salu_per_pair"] * n
+        per = NUM_CUS * CLOCK_GHZ * 1e9 * float(tier_ms[dom]) / 1000.0
+        out["valu_issue"] = {"kernel": dom_shape.get("kernel"), "valu_per_cu_clock": round(valu / per, 4),
+                             "salu_per_cu_clock": round(salu / per, 4), "frac": round(valu / per, 4),
+                             "from": sq["_path"]}
 
     if not args.no_cpu:
         t0 = time.time()
