@@ -489,25 +489,36 @@ DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h) {
   return imin(n, lim);
 }
 
-// Extension of two diagonals of a lane (the global tiers' vector steps call it
-// for diagonals 0-1 and 2-3 of their four): the first 32-base window of both is
-// loaded before either is compared, so their sequence reads overlap.  Returns
-// the matches within the first window (32 when it matches completely); the
-// caller continues those rare diagonals word by word.  (v, h) must be
-// in-bounds positions (callers pass 0, 0 for diagonals not extended).
+// Extension of two diagonals of a lane over their first 16 bases (the global
+// tiers' vector steps call it for diagonals 0-1 and 2-3 of their four).  The
+// sequences are read as 32-bit halves of the packed words, 16 bases each, and
+// one funnel shift forms a window from two consecutive halves: base j is half
+// j / 16, bits 2 * (j % 16).  All four reads are issued before either window is
+// formed.  Returns the matches within the window, 16 when it matches completely
+// (on random sequence almost never away from the alignment path); the caller
+// continues those diagonals word by word from base 16.  (v, h) must be
+// in-bounds positions (callers pass 0, 0 for diagonals not extended); the halves
+// read lie in the words an extension from (v, h) reads anyway.
+typedef const __attribute__((address_space(3))) uint32_t* lds_halves;
+typedef const __attribute__((address_space(1))) uint32_t* global_halves;
+
+DI uint32_t window16(uint32_t lo, uint32_t hi, int sh) {
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (sh << 1));
+}
+
 template <class Ptr>
 DI void lcp_first2(Ptr P, Ptr T, const Dir& d, int v0, int h0, int v1, int h1, int& n0, int& n1) {
   const bool rev = d.rev != 0;
-  const int pp0 = rev ? d.pb + d.plen - 32 - v0 : d.pb + v0;
-  const int tp0 = rev ? d.tb + d.tlen - 32 - h0 : d.tb + h0;
-  const int pp1 = rev ? d.pb + d.plen - 32 - v1 : d.pb + v1;
-  const int tp1 = rev ? d.tb + d.tlen - 32 - h1 : d.tb + h1;
-  const uint64_t a0 = P[pp0 >> 5], a1 = P[(pp0 >> 5) + 1], b0 = T[tp0 >> 5], b1 = T[(tp0 >> 5) + 1];
-  const uint64_t c0 = P[pp1 >> 5], c1 = P[(pp1 >> 5) + 1], e0 = T[tp1 >> 5], e1 = T[(tp1 >> 5) + 1];
-  const uint64_t x0 = window2(a0, a1, pp0 & 31) ^ window2(b0, b1, tp0 & 31);
-  const uint64_t x1 = window2(c0, c1, pp1 & 31) ^ window2(e0, e1, tp1 & 31);
-  n0 = x0 == 0 ? 32 : (rev ? __builtin_clzll(x0) : __builtin_ctzll(x0)) >> 1;
-  n1 = x1 == 0 ? 32 : (rev ? __builtin_clzll(x1) : __builtin_ctzll(x1)) >> 1;
+  const int pp0 = rev ? d.pb + d.plen - 16 - v0 : d.pb + v0;
+  const int tp0 = rev ? d.tb + d.tlen - 16 - h0 : d.tb + h0;
+  const int pp1 = rev ? d.pb + d.plen - 16 - v1 : d.pb + v1;
+  const int tp1 = rev ? d.tb + d.tlen - 16 - h1 : d.tb + h1;
+  const uint32_t a0 = P[pp0 >> 4], a1 = P[(pp0 >> 4) + 1], b0 = T[tp0 >> 4], b1 = T[(tp0 >> 4) + 1];
+  const uint32_t c0 = P[pp1 >> 4], c1 = P[(pp1 >> 4) + 1], e0 = T[tp1 >> 4], e1 = T[(tp1 >> 4) + 1];
+  const uint32_t x0 = window16(a0, a1, pp0 & 15) ^ window16(b0, b1, tp0 & 15);
+  const uint32_t x1 = window16(c0, c1, pp1 & 15) ^ window16(e0, e1, tp1 & 15);
+  n0 = x0 == 0 ? 16 : (rev ? __builtin_clz(x0) : __builtin_ctz(x0)) >> 1;
+  n1 = x1 == 0 ? 16 : (rev ? __builtin_clz(x1) : __builtin_ctz(x1)) >> 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -732,23 +743,23 @@ DI void step_tiles(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, 
       hh[j] = live[j] ? mx[j] : 0;
     }
     if (c.seq_lds) {
-      lcp_first2((lds_words)c.P, (lds_words)c.T, d, vv[0], hh[0], vv[1], hh[1], ext[0], ext[1]);
-      lcp_first2((lds_words)c.P, (lds_words)c.T, d, vv[2], hh[2], vv[3], hh[3], ext[2], ext[3]);
+      lcp_first2((lds_halves)c.P, (lds_halves)c.T, d, vv[0], hh[0], vv[1], hh[1], ext[0], ext[1]);
+      lcp_first2((lds_halves)c.P, (lds_halves)c.T, d, vv[2], hh[2], vv[3], hh[3], ext[2], ext[3]);
     } else {
-      lcp_first2((global_words)c.P, (global_words)c.T, d, vv[0], hh[0], vv[1], hh[1], ext[0], ext[1]);
-      lcp_first2((global_words)c.P, (global_words)c.T, d, vv[2], hh[2], vv[3], hh[3], ext[2], ext[3]);
+      lcp_first2((global_halves)c.P, (global_halves)c.T, d, vv[0], hh[0], vv[1], hh[1], ext[0], ext[1]);
+      lcp_first2((global_halves)c.P, (global_halves)c.T, d, vv[2], hh[2], vv[3], hh[3], ext[2], ext[3]);
     }
     // a diagonal whose first window matched completely continues word by word
     // (rare away from the alignment path); one diagonal at a time per lane
     int more = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      more |= (live[j] && ext[j] == 32 && imin(d.plen - vv[j], d.tlen - hh[j]) > 32) ? (1 << j) : 0;
+      more |= (live[j] && ext[j] == 16 && imin(d.plen - vv[j], d.tlen - hh[j]) > 16) ? (1 << j) : 0;
     while (more) {
       const int j = __builtin_ctz(more);
       more &= more - 1;
       const int v = pick4(vv, j), h = pick4(hh, j);
-      const int r = 32 + lcp(c, d, v + 32, h + 32);
+      const int r = 16 + lcp(c, d, v + 16, h + 16);
       ext[0] = j == 0 ? r : ext[0];
       ext[1] = j == 1 ? r : ext[1];
       ext[2] = j == 2 ? r : ext[2];
@@ -2235,9 +2246,12 @@ DI void task_loop(Ctx<K>& c, const KParams& p, int count, int32_t* lds, int wv) 
 // ---------------------------------------------------------------------------
 // kernels
 
+// Every instantiation is held to 128 vector registers, 4 waves per SIMD: the
+// team kernels then spill a few dozen registers outside their step loops, which
+// costs less than the latency the fourth wave hides.
 template <class K>
 __global__ __launch_bounds__(K::kNW > 1 ? K::kNW * kWave : kWave * kWavesPerBlock)
-__attribute__((amdgpu_waves_per_eu(K::kLds || K::kNW == 1 ? 4 : 3, 8)))
+__attribute__((amdgpu_waves_per_eu(4, 8)))
 void biwfa_persistent(KParams p) {
   extern __shared__ int32_t lds[];
   __shared__ int32_t tred[K::kNW > 1 ? 2 * K::kNW * kRedSlots : 1];
