@@ -427,28 +427,39 @@ DI void set_hdr(Ctx<K>& c, const Dir& d, int score, int slot, int comp, int lo, 
 // ---------------------------------------------------------------------------
 // sequences: 2-bit packed, base j at bits 2*(j%32) of word j/32
 
-// Bases [32w + sh, 32w + sh + 32) from the two words lo = word w, hi = word w+1.
-DI uint64_t window2(uint64_t lo, uint64_t hi, int sh) {
-  const int b = sh << 1;
-  return (lo >> b) | ((hi << 1) << (63 - b));
+// The sequences are read as 32-bit halves of the packed words, 16 bases each:
+// base j is half j / 16, bits 2 * (j % 16).  One funnel shift forms the 16-base
+// window starting at base 16w + sh from the halves w and w + 1.
+typedef const __attribute__((address_space(3))) uint32_t* lds_halves;
+typedef const __attribute__((address_space(1))) uint32_t* global_halves;
+
+DI uint32_t window16(uint32_t lo, uint32_t hi, int sh) {
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (sh << 1));
 }
 
-// Longest common extension from (v,h) of the aligner's sub-problem, 32 bases
-// per 64-bit compare, clamped to the sub-problem's end.  The four words of an
+// Longest common extension from (v,h) of the aligner's sub-problem, 16 bases
+// per 32-bit compare, clamped to the sub-problem's end.  The four halves of an
 // iteration are loaded before either window is formed, so the two sequence
-// reads overlap.
+// reads overlap.  Typed pointers keep the loads in the address space the
+// pair's words live in (generic loads would wait on both the LDS and the
+// memory counters).
+template <bool kOneExit, class Ptr>
+DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h);
+template <class Ptr>
+DI int lcp_words64(Ptr P, Ptr T, const Dir& d, int v, int h);
 typedef const __attribute__((address_space(3))) uint64_t* lds_words;
 typedef const __attribute__((address_space(1))) uint64_t* global_words;
 
-template <bool kOneExit, class Ptr>
-DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h);
-
-// Extension with the sequence loads in the address space the pair's words
-// live in (generic loads would wait on both the LDS and the memory counters).
+// Team kernels compare 32 bases per 64-bit step instead (measured faster there:
+// fewer loop trips on long matches, and no more registers than they spill now).
 template <class K>
 DI int lcp(const Ctx<K>& c, const Dir& d, int v, int h) {
-  if (K::kLds || c.seq_lds) return lcp_words<K::kLds>((lds_words)c.P, (lds_words)c.T, d, v, h);
-  return lcp_words<false>((global_words)c.P, (global_words)c.T, d, v, h);
+  if constexpr (K::kNW > 1) {
+    if (c.seq_lds) return lcp_words64((lds_words)c.P, (lds_words)c.T, d, v, h);
+    return lcp_words64((global_words)c.P, (global_words)c.T, d, v, h);
+  }
+  if (K::kLds || c.seq_lds) return lcp_words<K::kLds>((lds_halves)c.P, (lds_halves)c.T, d, v, h);
+  return lcp_words<false>((global_halves)c.P, (global_halves)c.T, d, v, h);
 }
 
 template <bool kOneExit, class Ptr>
@@ -457,32 +468,63 @@ DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h) {
   // forward: windows start at P[pb+v+n], T[tb+h+n]; reverse: windows end at
   // P[pe-1-v-n], T[te-1-h-n] and the first mismatch is the highest differing base
   const bool rev = d.rev != 0;
+  const int pp = rev ? d.pb + d.plen - 16 - v : d.pb + v;
+  const int tp = rev ? d.tb + d.tlen - 16 - h : d.tb + h;
+  // windows move 16 bases (one half) per iteration: the in-half shifts stay put
+  const int sp = pp & 15, st = tp & 15, dw = rev ? -1 : 1;
+  int wp = pp >> 4, wt = tp >> 4;
+  int n = 0;
+  for (;;) {
+    const uint32_t p0 = P[wp], p1 = P[wp + 1];
+    const uint32_t t0 = T[wt], t1 = T[wt + 1];
+    const uint32_t x = window16(p0, p1, sp) ^ window16(t0, t1, st);
+    if constexpr (kOneExit) {
+      // One exit test per window: a lane leaves at its first mismatch or past
+      // the limit (a mismatch-free window counts 16 bases), so the divergent
+      // loop carries a single break mask instead of one per exit (fewer scalar
+      // mask operations per window, in the kernel that is bound by issue).
+      const int z = x == 0 ? 32 : rev ? __builtin_clz(x) : __builtin_ctz(x);
+      n += z >> 1;
+      if (x != 0 || n >= lim) break;
+    } else {
+      if (x) {
+        n += (rev ? __builtin_clz(x) : __builtin_ctz(x)) >> 1;
+        break;
+      }
+      n += 16;
+      if (n >= lim) break;
+    }
+    wp += dw;
+    wt += dw;
+  }
+  return imin(n, lim);
+}
+
+// The same over 32-base windows of two 64-bit words (word w, bits 2 * (j % 32)).
+DI uint64_t window32(uint64_t lo, uint64_t hi, int sh) {
+  const int b = sh << 1;
+  return (lo >> b) | ((hi << 1) << (63 - b));
+}
+
+template <class Ptr>
+DI int lcp_words64(Ptr P, Ptr T, const Dir& d, int v, int h) {
+  const int lim = imin(d.plen - v, d.tlen - h);
+  const bool rev = d.rev != 0;
   const int pp = rev ? d.pb + d.plen - 32 - v : d.pb + v;
   const int tp = rev ? d.tb + d.tlen - 32 - h : d.tb + h;
-  // windows move 32 bases (one word) per iteration: the in-word shifts stay put
   const int sp = pp & 31, st = tp & 31, dw = rev ? -1 : 1;
   int wp = pp >> 5, wt = tp >> 5;
   int n = 0;
   for (;;) {
     const uint64_t p0 = P[wp], p1 = P[wp + 1];
     const uint64_t t0 = T[wt], t1 = T[wt + 1];
-    const uint64_t x = window2(p0, p1, sp) ^ window2(t0, t1, st);
-    if constexpr (kOneExit) {
-      // One exit test per word: a lane leaves at its first mismatch or past
-      // the limit (a mismatch-free word counts 32 bases), so the divergent
-      // loop carries a single break mask instead of one per exit (fewer scalar
-      // mask operations per word, in the kernel that is bound by issue).
-      const int z = x == 0 ? 64 : rev ? __builtin_clzll(x) : __builtin_ctzll(x);
-      n += z >> 1;
-      if (x != 0 || n >= lim) break;
-    } else {
-      if (x) {
-        n += (rev ? __builtin_clzll(x) : __builtin_ctzll(x)) >> 1;
-        break;
-      }
-      n += 32;
-      if (n >= lim) break;
+    const uint64_t x = window32(p0, p1, sp) ^ window32(t0, t1, st);
+    if (x) {
+      n += (rev ? __builtin_clzll(x) : __builtin_ctzll(x)) >> 1;
+      break;
     }
+    n += 32;
+    if (n >= lim) break;
     wp += dw;
     wt += dw;
   }
@@ -497,14 +539,7 @@ DI int lcp_words(Ptr P, Ptr T, const Dir& d, int v, int h) {
 // formed.  Returns the matches within the window, 16 when it matches completely
 // (on random sequence almost never away from the alignment path); the caller
 // continues those diagonals word by word from base 16.  (v, h) must be
-// in-bounds positions (callers pass 0, 0 for diagonals not extended); the halves
-// read lie in the words an extension from (v, h) reads anyway.
-typedef const __attribute__((address_space(3))) uint32_t* lds_halves;
-typedef const __attribute__((address_space(1))) uint32_t* global_halves;
-
-DI uint32_t window16(uint32_t lo, uint32_t hi, int sh) {
-  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (sh << 1));
-}
+// in-bounds positions (callers pass 0, 0 for diagonals not extended).
 
 template <class Ptr>
 DI void lcp_first2(Ptr P, Ptr T, const Dir& d, int v0, int h0, int v1, int h1, int& n0, int& n1) {
