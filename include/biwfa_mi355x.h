@@ -82,7 +82,7 @@ typedef enum {
 typedef enum {
   BW_SCHEDULE_INPUT_ORDER = 0,           /* pairs taken in input order                     */
   BW_SCHEDULE_LENGTH_SORTED = 1,         /* plain: longest (p+t) first                     */
-  BW_SCHEDULE_LEARNED = 2                /* by the cost model's predicted work, largest first */
+  BW_SCHEDULE_LEARNED = 2                /* tiers and order from the cost model's predicted work */
 } bw_schedule_t;
 
 typedef struct {
@@ -221,8 +221,9 @@ int bw_last_timing(const bw_aligner_t* aligner, float* total_ms, float* main_ms,
 /* Selects the pair order and starting tier of later bw_dbatch_align calls
  * (bw_schedule_t).  For BW_SCHEDULE_LEARNED, `predicted_score` holds one
  * predicted WFA score (penalty units, >= 0) per pair (host): pairs start in the
- * first tier whose ring is wide enough for it and are ordered by predicted work,
- * largest first.  The schedule never changes any pair's result. */
+ * first tier whose ring is wide enough for it; the single-wave tiers take them
+ * by predicted work, largest first, and the team tiers longest first.  The
+ * schedule never changes any pair's result. */
 int bw_set_schedule(bw_aligner_t* aligner, bw_dbatch_t* dbatch, int schedule,
                     const float* predicted_score);
 
