@@ -802,6 +802,9 @@ DI void step_tiles(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, 
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) ext[j] = imin(ext[j], imin(d.plen - vv[j], d.tlen - hh[j]));
+    // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
+    bool in_i[4], in_d[4];
+    bool all_in = true;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = k0 + j;
@@ -811,13 +814,23 @@ DI void step_tiles(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, 
       } else {
         mx[j] = kOffsetNull;
       }
-      // in-bounds flags for wavefront_compute_trim_ends (wavefront_compute.c:579-613)
-      const bool in_i = act[j] && (uint32_t)ins1[j] <= tl && (uint32_t)(ins1[j] - k) <= pl;
-      const bool in_d = act[j] && (uint32_t)del1[j] <= tl && (uint32_t)(del1[j] - k) <= pl;
-      acc.fm = imin(acc.fm, live[j] ? k : INT_MAX); acc.lm = imax(acc.lm, live[j] ? k : INT_MIN);
-      acc.fi = imin(acc.fi, in_i ? k : INT_MAX); acc.li = imax(acc.li, in_i ? k : INT_MIN);
-      acc.fd = imin(acc.fd, in_d ? k : INT_MAX); acc.ld = imax(acc.ld, in_d ? k : INT_MIN);
+      in_i[j] = act[j] && (uint32_t)ins1[j] <= tl && (uint32_t)(ins1[j] - k) <= pl;
+      in_d[j] = act[j] && (uint32_t)del1[j] <= tl && (uint32_t)(del1[j] - k) <= pl;
+      all_in = all_in && live[j] && in_i[j] && in_d[j];
       if (k == k_end && act[j]) acc.endv = (ce == CM) ? mx[j] : (ce == CI) ? ins1[j] : del1[j];
+    }
+    if (all_in) {   // every component in bounds on all four diagonals (interior tiles)
+      acc.fm = imin(acc.fm, k0); acc.lm = imax(acc.lm, k0 + 3);
+      acc.fi = imin(acc.fi, k0); acc.li = imax(acc.li, k0 + 3);
+      acc.fd = imin(acc.fd, k0); acc.ld = imax(acc.ld, k0 + 3);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = k0 + j;
+        acc.fm = imin(acc.fm, live[j] ? k : INT_MAX); acc.lm = imax(acc.lm, live[j] ? k : INT_MIN);
+        acc.fi = imin(acc.fi, in_i[j] ? k : INT_MAX); acc.li = imax(acc.li, in_i[j] ? k : INT_MIN);
+        acc.fd = imin(acc.fd, in_d[j] ? k : INT_MAX); acc.ld = imax(acc.ld, in_d[j] ? k : INT_MIN);
+      }
     }
     if (k0 <= hi) {   // whole groups: positions past hi (up to hi | 3) are never read
       const int o = ring ? (k0 & mask) : k0;
