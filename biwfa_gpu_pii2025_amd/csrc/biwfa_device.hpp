@@ -25,10 +25,10 @@ constexpr int kStGuard = -405;       // termination guard tripped (never in a co
 constexpr int kMaxScope = 32;          // max(o+e, x)+1 supported by the ring layout
 constexpr int kWave = 64;              // gfx950 wavefront size
 constexpr int kWavesPerBlock = 4;      // independent single-wave aligners per workgroup
-constexpr int kMaxTeam = 16;
+constexpr int kMaxTeam = 16;            // waves cooperating on one pair (one workgroup)
 // LDS-tier workspace capacities (compile-time in its kernel): base-case scores
 // (one header lane each), backtrace runs, recursion stack entries
-constexpr int kLdsHcap = 64, kLdsRcap = 128, kLdsScap = 16;           // waves cooperating on one pair (one workgroup)
+constexpr int kLdsHcap = 64, kLdsRcap = 128, kLdsScap = 16;
 constexpr int kRedSlots = 3 * kMaxScope;   // team reduction values per wave
 constexpr int kSubInts = 10;           // recursion stack entry (a sub-problem)
 constexpr int kItemInts = 12;          // shared recursion task: ready flag + sub-problem

@@ -2428,24 +2428,21 @@ void biwfa_persistent(KParams p) {
 #endif
 
 #if BW_GROUP(0)
-// Sets every tier's {queue, count} counters to their initial values (a kernel
-// rather than a memcpy/memset so that the launch sequence can be captured in a graph).
-// Sets every tier's counters for a new launch sequence and advances the call
+// Sets every tier's counters for a new launch sequence (a kernel rather than a
+// memcpy or memset, so that the sequence can be captured in a graph) and advances the call
 // counter at counters[n] (the ready-flag epochs of the sequence derive from
 // it, on the device, so a captured sequence replays with fresh epochs).
 __global__ void biwfa_reset(int32_t* counters, const int32_t* init, int n) {
-  const int i = threadIdx.x;
-  if (i < n) counters[i] = init[i];
-  if (i == 0) counters[n] = (counters[n] + 1) & 0x0fffffff;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = init[i];
+  if (threadIdx.x == 0) counters[n] = (counters[n] + 1) & 0x0fffffff;
 }
 
 // No-allocation path (bw_align_device_noalloc): zeroes its two tiers' counters
 // (the admit kernel counts the LDS tier's pairs) and advances the call counter
 // at counters[n].  A kernel, not a memset, so a caller may capture the calls.
 __global__ void biwfa_noalloc_reset(int32_t* counters, int n) {
-  const int i = threadIdx.x;
-  if (i < n) counters[i] = 0;
-  if (i == 0) counters[n] = (counters[n] + 1) & 0x0fffffff;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0;
+  if (threadIdx.x == 0) counters[n] = (counters[n] + 1) & 0x0fffffff;
 }
 
 // No-allocation path: admits every well-formed pair with p + t <= max_pt to the

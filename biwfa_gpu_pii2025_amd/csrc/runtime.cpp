@@ -302,6 +302,7 @@ constexpr int64_t kNaMaxPairs = 1 << 20;          // pairs per call
 constexpr int kNaMaxPT = 2040;                    // p + t of the longest pair it aligns
 constexpr int64_t kNaScratchBytes = 64ll << 20;   // 64 MiB per GPU
 constexpr int64_t kNaCounterInts = 64;            // 2 tiers x kQctlInts + call counter, padded
+static_assert(kNaCounterInts >= 2 * bw::kQctlInts + 1, "no-allocation counter area too small");
 static __device__ __align__(256) int32_t scratch[kNaScratchBytes / 4];
 
 // Device address of `scratch` on each device (hipGetSymbolAddress with the
