@@ -611,7 +611,7 @@ class WavefrontAligner:
     def autotune(self, packed, repeats=2, predicted_score=None, progress=None):
         """Times the sample under candidate tunings (bw_autotune); keeps and returns the best.
         With predicted scores (one per sample pair) the sample runs under the learned schedule.
-        `progress(str)` receives one line per candidate."""
+        `progress(str)` receives one line per timed run and one per candidate."""
         cb = None
         if progress is not None:
             cb = _PROGRESS_FN(lambda msg, user: progress(msg.decode()))

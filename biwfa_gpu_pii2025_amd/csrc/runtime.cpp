@@ -1275,6 +1275,10 @@ int bw_last_tier_shape(const bw_aligner_t* a, int tier, bw_tier_shape_t* shape) 
 
 // Runs the sample under tuning `t` (upload, one warm-up, `repeats` timed runs)
 // and returns the mean per-tier and total milliseconds.
+static void report(const bw_aligner_t* a, const std::string& msg) {
+  if (a->progress) a->progress(msg.c_str(), a->progress_user);
+}
+
 static int time_tuning(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, const bw_tuning_t& t,
                        int repeats, std::vector<float>* tier_ms, float* total_ms) {
   a->tuning = t;
@@ -1290,6 +1294,10 @@ static int time_tuning(bw_aligner_t* a, const bw_batch_t* sample, const float* p
   for (int r = 0; r <= repeats && st == 0; ++r) {
     st = bw_dbatch_align(a, b);
     if (st == 0) st = bw_synchronize(a);
+    // one line per run: a candidate on long pairs can take minutes
+    if (st == 0)
+      report(a, "autotune run " + std::to_string(r) + " of " + std::to_string(repeats) +
+                    (r == 0 ? " (warm-up): " : ": ") + std::to_string(a->last_total_ms) + " ms");
     if (st != 0 || r == 0) continue;
     for (int i = 0; i < a->last_ntiers; ++i) (*tier_ms)[i] += a->tier_ms[i] / repeats;
     *total_ms += a->last_total_ms / repeats;
@@ -1303,10 +1311,6 @@ int bw_set_progress(bw_aligner_t* a, bw_progress_fn fn, void* user) {
   a->progress = fn;
   a->progress_user = user;
   return 0;
-}
-
-static void report(const bw_aligner_t* a, const std::string& msg) {
-  if (a->progress) a->progress(msg.c_str(), a->progress_user);
 }
 
 // Smallest team that keeps a wave at no more than 256 chunks of 64 diagonals

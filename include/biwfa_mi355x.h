@@ -283,8 +283,8 @@ int bw_last_tier_shape(const bw_aligner_t* aligner, int tier, bw_tier_shape_t* s
 int bw_autotune(bw_aligner_t* aligner, const bw_batch_t* sample, const float* predicted_score,
                 int repeats, bw_tuning_t* best);
 
-/* Optional progress reports of long calls (bw_autotune: one line per candidate
- * timed).  fn may be NULL to switch them off. */
+/* Optional progress reports of long calls (bw_autotune: one line per timed run
+ * and one per candidate).  fn may be NULL to switch them off. */
 typedef void (*bw_progress_fn)(const char* message, void* user);
 int bw_set_progress(bw_aligner_t* aligner, bw_progress_fn fn, void* user);
 

@@ -879,11 +879,15 @@ static void align_batch(int n, const char* seqbuf,
                         int64_t* cells, int64_t* words, double* elapsed) {
   if (nthreads <= 0) nthreads = omp_get_max_threads();
   const double t0 = now_seconds();
+  /* chunks of 16 pairs (dynamic), fewer when that would leave a thread with
+   * under 64 chunks: a small sample of long pairs must still spread over
+   * every thread */
+  const int chunk = n / (64 * nthreads) >= 16 ? 16 : (n / (64 * nthreads) >= 1 ? n / (64 * nthreads) : 1);
   #pragma omp parallel num_threads(nthreads)
   {
     ctx_t c;
     memset(&c, 0, sizeof(c));
-    #pragma omp for schedule(dynamic, 16)
+    #pragma omp for schedule(dynamic, chunk)
     for (int i = 0; i < n; ++i) {
       memset(&c.cnt, 0, sizeof(c.cnt));
       int nr = 0, as = 0, ws = 0;

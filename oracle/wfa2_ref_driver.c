@@ -85,12 +85,16 @@ static void align_batch(
     double* elapsed) {
   if (nthreads <= 0) nthreads = omp_get_max_threads();
   const double t0 = now_seconds();
+  /* chunks of 16 pairs (dynamic), fewer when that would leave a thread with
+   * under 64 chunks: a small sample of long pairs must still spread over
+   * every thread */
+  const int chunk = n / (64 * nthreads) >= 16 ? 16 : (n / (64 * nthreads) >= 1 ? n / (64 * nthreads) : 1);
   #pragma omp parallel num_threads(nthreads)
   {
     wavefront_aligner_t* const wf = new_aligner(pen2, two_piece, score_only, max_steps);
     affine_penalties_t pen = {.match = pen2->match, .mismatch = pen2->mismatch,
                               .gap_opening = pen2->gap_opening1, .gap_extension = pen2->gap_extension1};
-    #pragma omp for schedule(dynamic, 16)
+    #pragma omp for schedule(dynamic, chunk)
     for (int i = 0; i < n; ++i) {
       const int st = wavefront_align(wf, seqbuf + p_off[i], p_len[i],
                                      seqbuf + t_off[i], t_len[i]);
