@@ -174,6 +174,8 @@ def main():
                     help="CPU baseline on every K-th pair (default: sized for --cpu-seconds)")
     ap.add_argument("--autotune", type=int, default=0, metavar="N",
                     help="tune launch shapes on an N-pair sample of the workload before timing")
+    ap.add_argument("--autotune-repeats", type=int, default=2, metavar="R",
+                    help="timed runs per autotune candidate (after one warm-up run)")
     ap.add_argument("--tuning", default=None, help="JSON launch tuning (bw_tuning_t fields)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the captured launch sequence (bw_set_graph); per-tier times read 0")
@@ -223,7 +225,7 @@ def main():
     if args.autotune > 0:
         t0 = time.time()
         sample = bw.generate_dataset(cfg, num_pairs=min(args.autotune, n), seed=cfg.seed + 1000).pack()
-        tuned = aligner.autotune(sample, repeats=2, progress=log)
+        tuned = aligner.autotune(sample, repeats=args.autotune_repeats, progress=log)
         log("autotuned on %d pairs in %.1f s: %s" % (len(sample), time.time() - t0, json.dumps(tuned)))
     db = aligner.upload(packed)
 
@@ -262,7 +264,7 @@ def main():
             tinfo["pairs"], train_s, tinfo["initial_mse"], tinfo["final_mse"]))
         if sample is not None:
             spred = model.predict_score(cm.batch_features(sample)).astype(np.float32)
-            aligner.autotune(sample, repeats=2, predicted_score=spred, progress=log)
+            aligner.autotune(sample, repeats=args.autotune_repeats, predicted_score=spred, progress=log)
         db = aligner.upload(packed)
         t0 = time.time()
         pred = cm.learned_schedule(db, model, packed)

@@ -1363,7 +1363,15 @@ int bw_autotune(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, in
   }
   cur = pick;
   // 2. LDS tier: queue chunk, then ring width (whole-batch time: the ring width
-  //    moves pairs between tiers)
+  //    moves pairs between tiers).  A sample with no pair in the LDS tier (long
+  //    pairs only) cannot tell these settings apart, so they are kept.
+  if (a->tier_pairs[0] == 0) {
+    report(a, "autotune lds tier: no sample pair starts there, its settings are kept");
+    a->tuning = cur;
+    a->schedule = sched0;
+    if (best) *best = cur;
+    return 0;
+  }
   float best_total = 3.4e38f;
   static const int kChunks[] = {1, 2, 4, 8, 16};
   pick = cur;

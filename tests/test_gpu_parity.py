@@ -343,6 +343,28 @@ def test_autotune_reports_and_keeps_results(aligners, oracle):
     assert sum(pairs) >= len(packed) and pairs[0] + sum(pairs[1:]) == sum(pairs)
     _compare(res, oref.alg_score, [oref.cigar(i) for i in range(len(packed))], "autotuned")
     db.free()
+
+
+def test_autotune_long_pairs_keeps_lds_settings(oracle):
+    """A sample with no pair in the LDS tier (10 kbp pairs) tunes the global tiers
+    only: one progress line per timed run, the LDS-tier settings left as they were,
+    and the tuned aligner still bit-exact against the oracle."""
+    _say("autotune long pairs")
+    ds = bw.generate_dataset(bw.CONFIGS["C3"], num_pairs=12)
+    packed = ds.pack()
+    oref = oracle.align(ds.pairs(), threads=16)
+    a = bw.WavefrontAligner(penalties=(0, 4, 6, 2))
+    lines = []
+    t = a.autotune(packed, repeats=1, progress=lines.append)
+    assert any("no sample pair starts there" in line for line in lines)
+    assert not any("lds queue_chunk" in line or "lds_ring_width" in line for line in lines)
+    # five team candidates, each a warm-up and one timed run
+    assert sum(1 for line in lines if line.startswith("autotune run")) == 10
+    assert t["lds_ring_width"] == 0 and t["queue_chunk"][0] == 0
+    res = a.align_batch(packed)
+    assert a.tier_pairs()[0] == 0
+    _compare(res, oref.alg_score, [oref.cigar(i) for i in range(len(packed))], "autotuned-long")
+    a.close()
     a.close()
 
 
