@@ -35,7 +35,8 @@ def test_profile_figures_load_only_for_their_workload():
     assert tr["kernel"].startswith("bw::biwfa_persistent<")
     assert bench.profile_json("traffic", "C4", 20000) is None
     assert bench.profile_json("traffic", "C2", 1000000)["workload"] == "C2"
-    assert bench.profile_json("work_counts", "C3", 10000) is None
+    assert bench.profile_json("work_counts", "C3", 10000)["workload"] == "C3"
+    assert bench.profile_json("work_counts", "C3", 5000) is None
     # SQ counters: per-cell figures for the team kernel, per-pair ones for the LDS kernel
     sq4 = bench.profile_json("sq", "C4", 200000)
     assert abs(sq4["valu_per_cell"] - sq4["sq_insts_valu"] / sq4["cells_per_step"]) < 1e-3
