@@ -592,6 +592,13 @@ static int check_lds_layout(const bw_aligner_t* a, const Tier& t0) {
     set_error("LDS-tier layout does not match the kernel's constants");
     return BW_STATUS_INTERNAL;
   }
+  // a workgroup of the LDS tier holds kWavesPerBlock waves' workspaces
+  if (t0.dyn_lds > (size_t)160 * 1024) {
+    set_error("LDS tier: ring width " + std::to_string(L0.wcap) + " with sequences staged up to " +
+              std::to_string(t0.seq_cap) + " bases needs " + std::to_string(t0.dyn_lds / 1024) +
+              " KiB per workgroup, more than a CU's 160 KiB (tune a narrower ring or a shorter lds_seq_cap)");
+    return BW_STATUS_INVALID_ARG;
+  }
   return 0;
 }
 

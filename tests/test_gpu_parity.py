@@ -321,6 +321,14 @@ def test_tuning_changes_speed_not_results(aligners, oracle):
     with pytest.raises(bw.BiwfaError):
         a.tuning = {"lds_ring_width": 48}
     a.close()
+    # accepted values whose LDS workspaces do not fit a CU together (penalties with a
+    # ring depth of 31 scores at ring width 64: 190 KiB per workgroup of 4 waves) are
+    # refused at upload, before any launch, with the sizes in the message
+    a = bw.WavefrontAligner(penalties=(0, 4, 20, 10))
+    a.tuning = {"lds_ring_width": 64}
+    with pytest.raises(bw.BiwfaError, match="more than a CU's 160 KiB"):
+        a.align_batch(bw.pack_pairs(pairs[:4]))
+    a.close()
 
 
 def test_autotune_reports_and_keeps_results(aligners, oracle):
