@@ -714,17 +714,6 @@ DI int4 mask4(int4 v, const Wf& w, int k0) {
 // an input's [lo, hi] are NULL; writes cover whole 4-offset groups, so the
 // caller guarantees they alias nothing live (ring: (hi|3) - (lo&~3) < wcap;
 // arena: the output allocation spans those groups).
-// A global-tier step over [lo, hi] in tiles of 256 diagonals: lane l of the
-// wave that owns a tile computes diagonals kt + 4l .. kt + 4l + 3
-// (wavefront_compute_affine_idm, wavefront_compute_affine.c:65-85, then the
-// extension) from one 16-byte read per input, with the diagonal-minus-one and
-// diagonal-plus-one inputs of its outer diagonals taken from the neighbouring
-// lanes (DPP wave shifts) and, at the tile edges, from one read each.  Tiles
-// start at lo & ~3 and go to the team's waves in turn.  Same results as the
-// one-diagonal-per-lane loop: identical arithmetic per diagonal; reads outside
-// an input's [lo, hi] are NULL; writes cover whole 4-offset groups, so the
-// caller guarantees they alias nothing live (ring: (hi|3) - (lo&~3) < wcap;
-// arena: the output allocation spans those groups).
 template <class K, bool kMax>
 DI void step_tiles(const Ctx<K>& c, const Dir& d, int lo, int hi, const Wf& mm, const Wf& mo, const Wf& ie,
                    const Wf& de, bool has_i, bool has_d, int bm, int bi, int bd, int mask, int wid, int nw,

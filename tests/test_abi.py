@@ -136,7 +136,7 @@ def test_c_stub_compiles_and_links_against_the_header(tmp_path):
 
 
 def test_integration_doc_shows_the_stub():
-    """The C stub in INTEGRATION.md is the file the tests compile."""
+    """The integration guide shows the same C stub the tests compile (tests/c_stub/alg_line.c)."""
     doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     src = open(os.path.join(ROOT, "tests", "c_stub", "alg_line.c")).read()
     body = src[src.index("int main"):]

@@ -509,7 +509,7 @@ def test_ready_guard_gives_up_and_launch_ends(oracle):
 
 
 def test_c_stub_prints_the_golden_alg_lines(tmp_path, utest_pairs):
-    """INTEGRATION.md's C stub, compiled against include/biwfa_mi355x.h and run as
+    """The integration guide's C stub (tests/c_stub/alg_line.c), compiled against include/biwfa_mi355x.h and run as
     its own process: its line for a utest pair equals WFA2's golden .alg line."""
     import subprocess
     from test_abi import build_c_stub
