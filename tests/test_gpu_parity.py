@@ -261,6 +261,26 @@ def test_schedules_and_determinism(aligners, oracle):
     db.free()
 
 
+def test_learned_schedule_merges_an_underfilled_tier(aligners, oracle):
+    """Predictions that straddle the 256/1024 boundary on like pairs would split them
+    over two sequential, under-filled single-wave launches; the learned schedule
+    hands the narrower tier's pairs to the wider one, which runs them all."""
+    _say("learned tier merge")
+    ds = bw.generate_dataset(bw.CONFIGS["C1"], num_pairs=300)
+    ref = oracle.align(ds.pairs(), threads=16)
+    a = aligners((0, 4, 6, 2))
+    db = a.upload(ds.pack())
+    # 100 -> first tier at least 1.25*100/2+8 = 70.5 wide (256); 500 -> 320.5 (1024)
+    db.set_schedule(bw.SCHEDULE_LEARNED, np.where(np.arange(300) % 2 == 0, 100.0, 500.0))
+    db.align()
+    res = db.download()
+    shapes = a.tier_shapes()
+    assert [s["ring_width"] for s in shapes[1:3]] == [256, 1024]
+    assert a.tier_pairs()[1:3] == [0, 300]
+    _compare(res, ref.alg_score, [ref.cigar(i) for i in range(300)], "learned-merge")
+    db.free()
+
+
 def test_spill_chain_under_wrong_predictions(aligners, oracle):
     """Long divergent pairs predicted at score 0 start in the smallest workspace tier
     and must climb the spill chain tier by tier; results stay bit-exact."""
