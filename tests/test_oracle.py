@@ -211,6 +211,36 @@ def test_oracle_long_pairs_trigger_recursion(oracle):
         assert r.cells[i] > 10_000_000
 
 
+def test_work_counters_closed_forms(oracle):
+    """The cell and word counters behind the bench's int32 roofline, on pairs whose
+    counts follow from the definitions: C = sum of hi - lo + 1 over compute steps
+    (null steps count nothing), W = floor(run / 32) + 1 per extended diagonal.
+      * identical, n <= 100: the base case ends at score 0 after one extension
+        of run n: C = 0, W = n // 32 + 1;
+      * one mismatch at m, n <= 100: s = 0 extends run m; s = 2 is null; s = 4 is
+        the one-diagonal step M[0] + 1 (C = 1), then extends run n - m - 1;
+      * identical, n > 100: the breakpoint search's first forward extension
+        reaches the end, the end-reached path (score 0 <= 500) falls back to the
+        base case, which extends again: C = 0, W = 2 * (n // 32 + 1)."""
+    rng = np.random.default_rng(3)
+
+    def rs(n):
+        return "".join(rng.choice(list("ACGT"), n))
+
+    def flip(s, m):
+        return s[:m] + ("A" if s[m] != "A" else "C") + s[m + 1:]
+    a80, a70, a200 = rs(80), rs(70), rs(200)
+    cases = [((a80, a80), 0, 80 // 32 + 1),
+             ((a70, a70), 0, 70 // 32 + 1),
+             ((a80, flip(a80, 40)), 1, (40 // 32 + 1) + (39 // 32 + 1)),
+             ((a80, flip(a80, 0)), 1, (0 // 32 + 1) + (79 // 32 + 1)),
+             ((a200, a200), 0, 2 * (200 // 32 + 1))]
+    r = oracle.align([c[0] for c in cases], (0, 4, 6, 2), threads=1)
+    for i, (_, cells, words) in enumerate(cases):
+        assert (int(r.cells[i]), int(r.words[i])) == (cells, words), i
+    assert [int(v) for v in r.alg_score] == [0, 0, -4, -4, 0]
+
+
 def test_planted_tie_break_mistake_is_caught(utest_pairs, tmp_path):
     """Swapping the backtrace preference of D-ext over D-open (a plausible slip)
     must fail the golden comparison -- the fixtures pin the tie-breaks."""
