@@ -4,8 +4,9 @@
 //   x0 = log2(p+1)/16, x1 = log2(t+1)/16, x2 = log2(|p-t|+1)/16,
 //   x3 = fraction of 16 pattern 12-mers (evenly spaced) found in the text near
 //        their expected position,
-// target y = log2(1 + score)/16.  Weights are stored as BF16 (round to nearest
-// even of the FP32 master copy) and every product is accumulated in FP32.  The
+// target y = log2(1 + score)/16.  Weights are stored in device memory as BF16
+// (16-bit, round to nearest even of the FP32 master copy), widened to FP32 when a
+// block stages them in LDS, and every product is accumulated in FP32.  The
 // prediction only orders pairs and picks their starting workspace tier; it
 // never changes an alignment result.
 //
@@ -98,10 +99,25 @@ __global__ __launch_bounds__(256) void features_kernel(const uint64_t* words, co
 // ---------------------------------------------------------------------------
 // MLP
 
-// Copies the FP32 master parameters rounded to BF16 (the stored weights).
-__global__ void to_bf16_kernel(const float* master, float* w, int n) {
+// The upper half of an FP32 value that bf16_round left with a zero lower half.
+__host__ __device__ inline uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return (uint16_t)(u >> 16);
+}
+// A BF16 weight widened to FP32 (exact).
+__host__ __device__ inline float bf16_value(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// Stores the FP32 master parameters rounded to BF16: the weights forward and
+// backward use.
+__global__ void to_bf16_kernel(const float* master, uint16_t* w, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) w[i] = bf16_round(master[i]);
+  if (i < n) w[i] = bf16_bits(bf16_round(master[i]));
 }
 
 // One sample through the network (weights in LDS, activations in registers:
@@ -127,9 +143,9 @@ __device__ inline void forward_one(const float* w, const float* x, float* h1, fl
   *y = o;
 }
 
-__global__ __launch_bounds__(256) void forward_kernel(const float* w_bf16, const float* X, int n, float* Y) {
+__global__ __launch_bounds__(256) void forward_kernel(const uint16_t* w_bf16, const float* X, int n, float* Y) {
   __shared__ float w[kParams];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = w_bf16[i];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = bf16_value(w_bf16[i]);
   __syncthreads();
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -140,7 +156,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const float* w_bf16, const
 
 // Gradient of the mean squared error over samples [0,n) w.r.t. the (BF16)
 // weights; block b writes its partial sum over samples [64b, 64b+64).
-__global__ __launch_bounds__(256) void backward_kernel(const float* w_bf16, const float* X, const float* Yt,
+__global__ __launch_bounds__(256) void backward_kernel(const uint16_t* w_bf16, const float* X, const float* Yt,
                                                        int n, float inv_n, float* partial) {
   __shared__ float w[kParams];
   __shared__ float sx[kSamplesPerBlock][kIn];
@@ -149,7 +165,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const float* w_bf16, cons
   __shared__ float sd1[kSamplesPerBlock][kH];   // dL/d(pre-activation of layer 1)
   __shared__ float sg3[kSamplesPerBlock];       // dL/dy
   __shared__ float sh2[kSamplesPerBlock][kH];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = w_bf16[i];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = bf16_value(w_bf16[i]);
   __syncthreads();
   const int base = blockIdx.x * kSamplesPerBlock;
   const int tid = threadIdx.x;
@@ -238,7 +254,7 @@ struct bw_mlp {
   int dev = 0;
   hipStream_t stream = nullptr;
   float* d_master = nullptr;
-  float* d_w = nullptr;       // BF16-rounded weights used by forward/backward
+  uint16_t* d_w = nullptr;    // BF16 weights (16-bit) used by forward/backward
   float* d_m = nullptr;
   float* d_v = nullptr;
   float* d_grad = nullptr;
@@ -305,7 +321,7 @@ bw_mlp_t* bw_mlp_new(int device, uint64_t seed) {
   init_params(w, seed);
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&m->d_master), 4 * kParams) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void**>(&m->d_w), 4 * kParams) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&m->d_w), sizeof(uint16_t) * kParams) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&m->d_m), 4 * kParams) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&m->d_v), 4 * kParams) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&m->d_grad), 4 * kParams) != hipSuccess ||
@@ -322,8 +338,9 @@ bw_mlp_t* bw_mlp_new(int device, uint64_t seed) {
 void bw_mlp_free(bw_mlp_t* m) {
   if (!m) return;
   (void)hipSetDevice(m->dev);
-  for (float* p : {m->d_master, m->d_w, m->d_m, m->d_v, m->d_grad})
+  for (float* p : {m->d_master, m->d_m, m->d_v, m->d_grad})
     if (p) (void)hipFree(p);
+  if (m->d_w) (void)hipFree(m->d_w);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -334,7 +351,11 @@ int bw_mlp_get_params(bw_mlp_t* m, float* master, float* bf16_weights) {
   if (!m) return BW_STATUS_INVALID_ARG;
   CM_TRY(hipSetDevice(m->dev));
   if (master) CM_TRY(hipMemcpy(master, m->d_master, 4 * kParams, hipMemcpyDeviceToHost));
-  if (bf16_weights) CM_TRY(hipMemcpy(bf16_weights, m->d_w, 4 * kParams, hipMemcpyDeviceToHost));
+  if (bf16_weights) {
+    std::vector<uint16_t> b(kParams);
+    CM_TRY(hipMemcpy(b.data(), m->d_w, sizeof(uint16_t) * kParams, hipMemcpyDeviceToHost));
+    for (int i = 0; i < kParams; ++i) bf16_weights[i] = bf16_value(b[i]);
+  }
   return 0;
 }
 

@@ -354,7 +354,8 @@ int bw_mlp_num_params(void);
 bw_mlp_t* bw_mlp_new(int device, uint64_t seed);
 void bw_mlp_free(bw_mlp_t* mlp);
 const char* bw_mlp_last_error(void);
-/* FP32 master parameters and their BF16-rounded copy (either may be NULL). */
+/* FP32 master parameters and the BF16 weights (stored as 16-bit on the device;
+ * returned widened to FP32, exactly).  Either pointer may be NULL. */
 int bw_mlp_get_params(bw_mlp_t* mlp, float* master, float* bf16_weights);
 int bw_mlp_set_params(bw_mlp_t* mlp, const float* master);
 /* X: n x 4 features (host), y: n outputs (host). */
