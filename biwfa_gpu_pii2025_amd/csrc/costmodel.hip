@@ -11,11 +11,13 @@
 // never changes an alignment result.
 //
 // Kernels: feature extraction (one wave per pair over the 2-bit packed batch),
-// batched forward, per-block backward partial sums reduced in a fixed order
-// (deterministic gradients), and an Adam step.
+// batched forward (the 64 x 64 hidden layer on FP32 MFMA tiles), per-block
+// backward partial sums reduced in a fixed order (deterministic gradients), and
+// an Adam step.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -121,7 +123,8 @@ __global__ void to_bf16_kernel(const float* master, uint16_t* w, int n) {
 }
 
 // One sample through the network (weights in LDS, activations in registers:
-// every loop is fully unrolled so that h1/h2 never leave the register file).
+// every loop is fully unrolled so that h1/h2 never leave the register file); the
+// backward kernel's forward pass.
 __device__ inline void forward_one(const float* w, const float* x, float* h1, float* h2, float* y) {
 #pragma unroll
   for (int j = 0; j < kH; ++j) {
@@ -143,15 +146,84 @@ __device__ inline void forward_one(const float* w, const float* x, float* h1, fl
   *y = o;
 }
 
-__global__ __launch_bounds__(256) void forward_kernel(const uint16_t* w_bf16, const float* X, int n, float* Y) {
-  __shared__ float w[kParams];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = bf16_value(w_bf16[i]);
+// The same forward pass with the 64 x 64 hidden layer on the matrix cores: each
+// wave takes 32 samples at a time; layer 1 and the output layer run per lane in
+// forward_one's order, and layer 2 is two 32 x 32 FP32 MFMA tiles (32 steps of
+// v_mfma_f32_32x32x2f32 each).  Lane l holds A[l % 32][l / 32] and
+// B[l / 32][l % 32]; accumulator element v of lane l is row
+// (v / 4) * 8 + (l / 32) * 4 + v % 4, column l % 32.  Products and sums stay FP32
+// (only the summation order of layer 2 differs from forward_one).
+constexpr int kMfmaWaves = 4;                 // waves per block
+constexpr int kTileRows = 32;                 // samples per wave tile
+constexpr int kPad = kH + 1;                  // padded row: conflict-free column reads
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(64 * kMfmaWaves) void forward_mfma_kernel(const uint16_t* w_bf16, const float* X,
+                                                                       int n, float* Y) {
+  __shared__ float w1[kH * kIn], b1[kH], b2[kH], w3[kH], b3;
+  __shared__ float w2[kH * kPad];                         // [out][in], padded rows
+  __shared__ float tile[kMfmaWaves][kTileRows * kPad];    // h1, then h2, of a wave's samples
+  for (int i = threadIdx.x; i < kH * kIn; i += blockDim.x) w1[i] = bf16_value(w_bf16[kW1 + i]);
+  for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+    b1[i] = bf16_value(w_bf16[kB1 + i]);
+    b2[i] = bf16_value(w_bf16[kB2 + i]);
+    w3[i] = bf16_value(w_bf16[kW3 + i]);
+  }
+  for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) w2[(i / kH) * kPad + i % kH] = bf16_value(w_bf16[kW2 + i]);
+  if (threadIdx.x == 0) b3 = bf16_value(w_bf16[kB3]);
   __syncthreads();
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  float h1[kH], h2[kH], y;
-  forward_one(w, X + (size_t)s * kIn, h1, h2, &y);
-  Y[s] = y;
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r = l & 31, half = l >> 5;
+  float* t = tile[wv];
+  for (int64_t s0 = ((int64_t)blockIdx.x * kMfmaWaves + wv) * kTileRows; s0 < n;
+       s0 += (int64_t)gridDim.x * kMfmaWaves * kTileRows) {
+    // layer 1: lane (r, half) computes units half*32 .. half*32+31 of sample s0+r
+    float x[kIn] = {0.f, 0.f, 0.f, 0.f};
+    if (s0 + r < n) {
+#pragma unroll
+      for (int c = 0; c < kIn; ++c) x[c] = X[(size_t)(s0 + r) * kIn + c];
+    }
+    for (int u = half * 32; u < half * 32 + 32; ++u) {
+      float a = b1[u];
+#pragma unroll
+      for (int c = 0; c < kIn; ++c) a = fmaf(w1[u * kIn + c], x[c], a);
+      t[r * kPad + u] = a > 0.f ? a : 0.f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 2: D[sample][out] = sum_k h1[sample][k] * W2[out][k], two column blocks
+    f32x16 acc0 = {}, acc1 = {};
+    for (int kk = 0; kk < kH / 2; ++kk) {
+      const int k = 2 * kk + half;
+      const float a = t[r * kPad + k];
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w2[r * kPad + k], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w2[(32 + r) * kPad + k], acc1, 0, 0, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // bias + ReLU into the tile (h1 is no longer read)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int row = (v / 4) * 8 + half * 4 + (v % 4);
+      const float z0 = acc0[v] + b2[r], z1 = acc1[v] + b2[32 + r];
+      t[row * kPad + r] = z0 > 0.f ? z0 : 0.f;
+      t[row * kPad + 32 + r] = z1 > 0.f ? z1 : 0.f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // output layer in forward_one's order, one sample per lane of the first half
+    if (half == 0 && s0 + r < n) {
+      float o = b3;
+      for (int i = 0; i < kH; ++i) o = fmaf(w3[i], t[r * kPad + i], o);
+      Y[s0 + r] = o;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 }
 
 // Gradient of the mean squared error over samples [0,n) w.r.t. the (BF16)
@@ -282,6 +354,13 @@ struct DevBuf {
   hipError_t alloc(size_t n) { return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (n ? n : 1)); }
 };
 
+// Blocks of the forward kernel: one 32-sample tile per wave, at most 4096 blocks
+// (the kernel strides over the rest).
+unsigned forward_blocks(int64_t n) {
+  const int64_t tiles = (n + kTileRows - 1) / kTileRows;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + kMfmaWaves - 1) / kMfmaWaves, 4096));
+}
+
 int refresh_bf16(bw_mlp* m) {
   hipLaunchKernelGGL(to_bf16_kernel, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, m->d_master,
                      m->d_w, kParams);
@@ -378,8 +457,8 @@ int bw_mlp_predict(bw_mlp_t* m, const float* X, int64_t n, float* y) {
   CM_TRY(dX.alloc(n * kIn));
   CM_TRY(dY.alloc(n));
   CM_TRY(hipMemcpyAsync(dX.p, X, 4 * n * kIn, hipMemcpyHostToDevice, m->stream));
-  hipLaunchKernelGGL(forward_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->d_w, dX.p,
-                     (int)n, dY.p);
+  hipLaunchKernelGGL(forward_mfma_kernel, dim3(forward_blocks(n)), dim3(64 * kMfmaWaves), 0, m->stream, m->d_w,
+                     dX.p, (int)n, dY.p);
   CM_TRY(hipGetLastError());
   CM_TRY(hipMemcpyAsync(y, dY.p, 4 * n, hipMemcpyDeviceToHost, m->stream));
   CM_TRY(hipStreamSynchronize(m->stream));
@@ -460,7 +539,7 @@ int bw_mlp_train(bw_mlp_t* m, const float* X, const float* y, int64_t n, int epo
   if (final_loss) {
     std::vector<float> pred(n);
     CM_TRY(hipMemcpyAsync(dX.p, X, 4 * n * kIn, hipMemcpyHostToDevice, m->stream));
-    hipLaunchKernelGGL(forward_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->d_w,
+    hipLaunchKernelGGL(forward_mfma_kernel, dim3(forward_blocks(n)), dim3(64 * kMfmaWaves), 0, m->stream, m->d_w,
                        dX.p, (int)n, dPred.p);
     CM_TRY(hipGetLastError());
     CM_TRY(hipMemcpyAsync(pred.data(), dPred.p, 4 * n, hipMemcpyDeviceToHost, m->stream));

@@ -152,10 +152,14 @@ def test_device_forward_and_bf16_weights_match_oracle(cm):
     model = cm.CostModel(seed=7)
     master, w = model.params()
     assert np.array_equal(w, mo.bf16_round(master))
-    X = np.random.default_rng(8).uniform(0, 1, (3000, 4)).astype(np.float32)
-    y = model.predict(X)
-    ref, _ = mo.forward(w.astype(np.float64), X.astype(np.float64), np.float64)
-    assert np.max(np.abs(y - ref)) <= 1e-5 * max(1.0, np.max(np.abs(ref)))
+    # the forward kernel runs 32-sample tiles per wave (hidden layer on FP32 MFMA):
+    # whole and partial tiles, FP32 against the float64 restatement
+    rng = np.random.default_rng(8)
+    for n in (1, 31, 33, 3000):
+        X = rng.uniform(0, 1, (n, 4)).astype(np.float32)
+        y = model.predict(X)
+        ref, _ = mo.forward(w.astype(np.float64), X.astype(np.float64), np.float64)
+        assert np.max(np.abs(y - ref)) <= 1e-5 * max(1.0, np.max(np.abs(ref))), n
     model.close()
 
 
