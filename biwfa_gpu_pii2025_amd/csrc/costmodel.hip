@@ -11,9 +11,9 @@
 // never changes an alignment result.
 //
 // Kernels: feature extraction (one wave per pair over the 2-bit packed batch),
-// batched forward (the 64 x 64 hidden layer on FP32 MFMA tiles), per-block
-// backward partial sums reduced in a fixed order (deterministic gradients), and
-// an Adam step.
+// batched forward and per-block backward partial sums (the 64 x 64 products on
+// FP32 MFMA tiles) reduced in a fixed order (deterministic gradients), and an
+// Adam step.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -36,7 +36,6 @@ constexpr int kB2 = kW2 + kH * kH;     // [kH]
 constexpr int kW3 = kB2 + kH;          // [kH]
 constexpr int kB3 = kW3 + kH;          // [1]
 constexpr int kParams = kB3 + 1;
-constexpr int kSamplesPerBlock = 64;   // backward block tile
 
 // FP32 -> BF16 (round to nearest, ties to even) -> FP32
 __host__ __device__ inline float bf16_round(float f) {
@@ -122,37 +121,13 @@ __global__ void to_bf16_kernel(const float* master, uint16_t* w, int n) {
   if (i < n) w[i] = bf16_bits(bf16_round(master[i]));
 }
 
-// One sample through the network (weights in LDS, activations in registers:
-// every loop is fully unrolled so that h1/h2 never leave the register file); the
-// backward kernel's forward pass.
-__device__ inline void forward_one(const float* w, const float* x, float* h1, float* h2, float* y) {
-#pragma unroll
-  for (int j = 0; j < kH; ++j) {
-    float a = w[kB1 + j];
-#pragma unroll
-    for (int i = 0; i < kIn; ++i) a = fmaf(w[kW1 + j * kIn + i], x[i], a);
-    h1[j] = a > 0.f ? a : 0.f;
-  }
-#pragma unroll
-  for (int j = 0; j < kH; ++j) {
-    float a = w[kB2 + j];
-#pragma unroll
-    for (int i = 0; i < kH; ++i) a = fmaf(w[kW2 + j * kH + i], h1[i], a);
-    h2[j] = a > 0.f ? a : 0.f;
-  }
-  float o = w[kB3];
-#pragma unroll
-  for (int i = 0; i < kH; ++i) o = fmaf(w[kW3 + i], h2[i], o);
-  *y = o;
-}
-
-// The same forward pass with the 64 x 64 hidden layer on the matrix cores: each
-// wave takes 32 samples at a time; layer 1 and the output layer run per lane in
-// forward_one's order, and layer 2 is two 32 x 32 FP32 MFMA tiles (32 steps of
-// v_mfma_f32_32x32x2f32 each).  Lane l holds A[l % 32][l / 32] and
+// The forward pass, with the 64 x 64 hidden layer on the matrix cores: each wave
+// takes 32 samples at a time; layer 1 and the output layer run per lane, each
+// unit's sum in input order, and layer 2 is two 32 x 32 FP32 MFMA tiles (32 steps
+// of v_mfma_f32_32x32x2f32 each).  Lane l holds A[l % 32][l / 32] and
 // B[l / 32][l % 32]; accumulator element v of lane l is row
 // (v / 4) * 8 + (l / 32) * 4 + v % 4, column l % 32.  Products and sums stay FP32
-// (only the summation order of layer 2 differs from forward_one).
+// (the MFMA fixes layer 2's summation order).
 constexpr int kMfmaWaves = 4;                 // waves per block
 constexpr int kTileRows = 32;                 // samples per wave tile
 constexpr int kPad = kH + 1;                  // padded row: conflict-free column reads
@@ -214,7 +189,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void forward_mfma_kernel(const uin
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // output layer in forward_one's order, one sample per lane of the first half
+    // output layer, one sample per lane of the first half
     if (half == 0 && s0 + r < n) {
       float o = b3;
       for (int i = 0; i < kH; ++i) o = fmaf(w3[i], t[r * kPad + i], o);
@@ -227,71 +202,133 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void forward_mfma_kernel(const uin
 }
 
 // Gradient of the mean squared error over samples [0,n) w.r.t. the (BF16)
-// weights; block b writes its partial sum over samples [64b, 64b+64).
+// weights; block b writes its partial sum over samples [64b, 64b+64).  Four
+// waves; the three 64 x 64 products run as 32 x 32 FP32 MFMA tiles (lane layout
+// as in forward_mfma_kernel):
+//   forward   h2 = relu(h1 W2^T + b2)       waves 0-1, 32 samples each
+//   backward  d1 = (d2 W2) masked by h1 > 0  one 32 x 32 tile per wave
+//   gradient  dW2 = d2^T h1                 one 32 x 32 tile per wave
+// where d2 = dL/d(pre-activation of layer 2).  Layer 1, the output layer and the
+// bias and small-weight sums run per lane in sample order.  Deterministic: the
+// same inputs give the same bits.
+constexpr int kBwdRows = 64;                  // samples per block
 __global__ __launch_bounds__(256) void backward_kernel(const uint16_t* w_bf16, const float* X, const float* Yt,
                                                        int n, float inv_n, float* partial) {
-  __shared__ float w[kParams];
-  __shared__ float sx[kSamplesPerBlock][kIn];
-  __shared__ float sh1[kSamplesPerBlock][kH];
-  __shared__ float sd2[kSamplesPerBlock][kH];   // dL/d(pre-activation of layer 2)
-  __shared__ float sd1[kSamplesPerBlock][kH];   // dL/d(pre-activation of layer 1)
-  __shared__ float sg3[kSamplesPerBlock];       // dL/dy
-  __shared__ float sh2[kSamplesPerBlock][kH];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = bf16_value(w_bf16[i]);
+  __shared__ float w1[kH * kIn], b1[kH], w3[kH], b2[kH], b3;
+  __shared__ float w2[kH * kPad];                 // [out][in], padded rows
+  __shared__ float sx[kBwdRows][kIn];
+  __shared__ float sh1[kBwdRows * kPad];          // layer-1 activations
+  __shared__ float sh2[kBwdRows * kPad];          // layer-2 activations
+  __shared__ float sd2[kBwdRows * kPad];          // dL/d(pre-activation of layer 2)
+  __shared__ float sd1[kBwdRows * kPad];          // dL/d(pre-activation of layer 1)
+  __shared__ float sg3[kBwdRows];                 // dL/dy
+  for (int i = threadIdx.x; i < kH * kIn; i += blockDim.x) w1[i] = bf16_value(w_bf16[kW1 + i]);
+  for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+    b1[i] = bf16_value(w_bf16[kB1 + i]);
+    b2[i] = bf16_value(w_bf16[kB2 + i]);
+    w3[i] = bf16_value(w_bf16[kW3 + i]);
+  }
+  for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) w2[(i / kH) * kPad + i % kH] = bf16_value(w_bf16[kW2 + i]);
+  if (threadIdx.x == 0) b3 = bf16_value(w_bf16[kB3]);
   __syncthreads();
-  const int base = blockIdx.x * kSamplesPerBlock;
-  const int tid = threadIdx.x;
-  if (tid < kSamplesPerBlock) {
-    const int s = base + tid;
-    float h1[kH], h2[kH], y = 0.f;
+  const int base = blockIdx.x * kBwdRows;
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, r = l & 31, half = l >> 5;
+  if (wv < 2) {   // forward of samples base + 32 wv + (0 .. 31)
+    const int s = 32 * wv + r;
     float x[kIn] = {0.f, 0.f, 0.f, 0.f};
-    if (s < n) {
+    if (base + s < n) {
 #pragma unroll
-      for (int i = 0; i < kIn; ++i) x[i] = X[(size_t)s * kIn + i];
-      forward_one(w, x, h1, h2, &y);
-    } else {
-#pragma unroll
-      for (int i = 0; i < kH; ++i) { h1[i] = 0.f; h2[i] = 0.f; }
+      for (int c = 0; c < kIn; ++c) x[c] = X[(size_t)(base + s) * kIn + c];
     }
-    const float g = (s < n) ? 2.0f * (y - Yt[s]) * inv_n : 0.0f;
-    sg3[tid] = g;
+    if (half == 0) {
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) sx[tid][i] = x[i];
+      for (int c = 0; c < kIn; ++c) sx[s][c] = x[c];
+    }
+    for (int u = half * 32; u < half * 32 + 32; ++u) {
+      float a = b1[u];
 #pragma unroll
-    for (int j = 0; j < kH; ++j) {
-      sh1[tid][j] = h1[j];
-      sh2[tid][j] = h2[j];
-      sd2[tid][j] = (h2[j] > 0.f) ? g * w[kW3 + j] : 0.f;
+      for (int c = 0; c < kIn; ++c) a = fmaf(w1[u * kIn + c], x[c], a);
+      sh1[s * kPad + u] = a > 0.f ? a : 0.f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    f32x16 acc0 = {}, acc1 = {};
+    for (int kk = 0; kk < kH / 2; ++kk) {
+      const int k = 2 * kk + half;
+      const float a = sh1[(32 * wv + r) * kPad + k];
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w2[r * kPad + k], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w2[(32 + r) * kPad + k], acc1, 0, 0, 0);
     }
 #pragma unroll
-    for (int i = 0; i < kH; ++i) {
-      float a = 0.f;
-      for (int j = 0; j < kH; ++j) a = fmaf(w[kW2 + j * kH + i], sd2[tid][j], a);
-      sd1[tid][i] = (h1[i] > 0.f) ? a : 0.f;
+    for (int v = 0; v < 16; ++v) {
+      const int row = 32 * wv + (v / 4) * 8 + half * 4 + (v % 4);
+      const float z0 = acc0[v] + b2[r], z1 = acc1[v] + b2[32 + r];
+      sh2[row * kPad + r] = z0 > 0.f ? z0 : 0.f;
+      sh2[row * kPad + 32 + r] = z1 > 0.f ? z1 : 0.f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (half == 0) {   // output layer and dL/dy of sample s
+      float o = b3;
+      for (int i = 0; i < kH; ++i) o = fmaf(w3[i], sh2[s * kPad + i], o);
+      sg3[s] = (base + s < n) ? 2.0f * (o - Yt[base + s]) * inv_n : 0.0f;
     }
   }
   __syncthreads();
+  for (int q = tid; q < kBwdRows * kH; q += blockDim.x) {
+    const int s = q / kH, j = q % kH;
+    sd2[s * kPad + j] = (sh2[s * kPad + j] > 0.f) ? sg3[s] * w3[j] : 0.f;
+  }
+  __syncthreads();
+  const int rb = wv & 1, cb = wv >> 1;   // this wave's 32 x 32 tile: row block, column block
+  {  // d1[s][i] = sum_j d2[s][j] W2[j][i], masked by h1[s][i] > 0
+    f32x16 acc = {};
+    for (int kk = 0; kk < kH / 2; ++kk) {
+      const int k = 2 * kk + half;
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sd2[(32 * rb + r) * kPad + k], w2[k * kPad + 32 * cb + r], acc, 0,
+                                                 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int sr = 32 * rb + (v / 4) * 8 + half * 4 + (v % 4), i = 32 * cb + r;
+      sd1[sr * kPad + i] = (sh1[sr * kPad + i] > 0.f) ? acc[v] : 0.f;
+    }
+  }
   float* out = partial + (size_t)blockIdx.x * kParams;
-  // every parameter's partial sum over the block's samples, in sample order
+  {  // dW2[j][i] = sum_s d2[s][j] h1[s][i]
+    f32x16 acc = {};
+    for (int kk = 0; kk < kBwdRows / 2; ++kk) {
+      const int k = 2 * kk + half;
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sd2[k * kPad + 32 * rb + r], sh1[k * kPad + 32 * cb + r], acc, 0,
+                                                 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int j = 32 * rb + (v / 4) * 8 + half * 4 + (v % 4), i = 32 * cb + r;
+      out[kW2 + j * kH + i] = acc[v];
+    }
+  }
+  __syncthreads();
+  // the other parameters' partial sums over the block's samples, in sample order
   for (int q = tid; q < kParams; q += blockDim.x) {
+    if (q >= kW2 && q < kB2) continue;
     float acc = 0.f;
     if (q < kB1) {
       const int j = (q - kW1) / kIn, i = (q - kW1) % kIn;
-      for (int s = 0; s < kSamplesPerBlock; ++s) acc = fmaf(sd1[s][j], sx[s][i], acc);
+      for (int s = 0; s < kBwdRows; ++s) acc = fmaf(sd1[s * kPad + j], sx[s][i], acc);
     } else if (q < kW2) {
       const int j = q - kB1;
-      for (int s = 0; s < kSamplesPerBlock; ++s) acc += sd1[s][j];
-    } else if (q < kB2) {
-      const int j = (q - kW2) / kH, i = (q - kW2) % kH;
-      for (int s = 0; s < kSamplesPerBlock; ++s) acc = fmaf(sd2[s][j], sh1[s][i], acc);
+      for (int s = 0; s < kBwdRows; ++s) acc += sd1[s * kPad + j];
     } else if (q < kW3) {
       const int j = q - kB2;
-      for (int s = 0; s < kSamplesPerBlock; ++s) acc += sd2[s][j];
+      for (int s = 0; s < kBwdRows; ++s) acc += sd2[s * kPad + j];
     } else if (q < kB3) {
       const int i = q - kW3;
-      for (int s = 0; s < kSamplesPerBlock; ++s) acc = fmaf(sg3[s], sh2[s][i], acc);
+      for (int s = 0; s < kBwdRows; ++s) acc = fmaf(sg3[s], sh2[s * kPad + i], acc);
     } else {
-      for (int s = 0; s < kSamplesPerBlock; ++s) acc += sg3[s];
+      for (int s = 0; s < kBwdRows; ++s) acc += sg3[s];
     }
     out[q] = acc;
   }
@@ -469,7 +506,7 @@ int bw_mlp_predict(bw_mlp_t* m, const float* X, int64_t n, float* y) {
 int bw_mlp_gradient(bw_mlp_t* m, const float* X, const float* y, int64_t n, float* grad) {
   if (!m || n <= 0 || !X || !y || !grad) return BW_STATUS_INVALID_ARG;
   CM_TRY(hipSetDevice(m->dev));
-  const int blocks = (int)((n + kSamplesPerBlock - 1) / kSamplesPerBlock);
+  const int blocks = (int)((n + kBwdRows - 1) / kBwdRows);
   DevBuf<float> dX, dY, dP;
   CM_TRY(dX.alloc(n * kIn));
   CM_TRY(dY.alloc(n));
@@ -497,7 +534,7 @@ int bw_mlp_train(bw_mlp_t* m, const float* X, const float* y, int64_t n, int epo
   CM_TRY(dY.alloc(n));
   CM_TRY(dXb.alloc((size_t)batch * kIn));
   CM_TRY(dYb.alloc(batch));
-  const int max_blocks = (batch + kSamplesPerBlock - 1) / kSamplesPerBlock;
+  const int max_blocks = (batch + kBwdRows - 1) / kBwdRows;
   CM_TRY(dP.alloc((size_t)max_blocks * kParams));
   CM_TRY(dPred.alloc(n));
   // host-side shuffled copies per epoch (the permutation is cheap; the math is on the device)
@@ -521,7 +558,7 @@ int bw_mlp_train(bw_mlp_t* m, const float* X, const float* y, int64_t n, int epo
       }
       CM_TRY(hipMemcpyAsync(dXb.p, hx.data(), 4 * (size_t)nb * kIn, hipMemcpyHostToDevice, m->stream));
       CM_TRY(hipMemcpyAsync(dYb.p, hy.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, m->stream));
-      const int blocks = (nb + kSamplesPerBlock - 1) / kSamplesPerBlock;
+      const int blocks = (nb + kBwdRows - 1) / kBwdRows;
       hipLaunchKernelGGL(backward_kernel, dim3(blocks), dim3(256), 0, m->stream, m->d_w, dXb.p, dYb.p, nb,
                          1.0f / (float)nb, dP.p);
       hipLaunchKernelGGL(reduce_kernel, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, dP.p, blocks,
