@@ -1298,12 +1298,12 @@ int bw_last_tier_shape(const bw_aligner_t* a, int tier, bw_tier_shape_t* shape) 
   return 0;
 }
 
-// Runs the sample under tuning `t` (upload, one warm-up, `repeats` timed runs)
-// and returns the mean per-tier and total milliseconds.
 static void report(const bw_aligner_t* a, const std::string& msg) {
   if (a->progress) a->progress(msg.c_str(), a->progress_user);
 }
 
+// Runs the sample under tuning `t` (upload, one warm-up, `repeats` timed runs)
+// and returns the mean per-tier and total milliseconds.
 static int time_tuning(bw_aligner_t* a, const bw_batch_t* sample, const float* pred, const bw_tuning_t& t,
                        int repeats, std::vector<float>* tier_ms, float* total_ms) {
   a->tuning = t;
