@@ -94,7 +94,7 @@ typedef struct {
   int waves_per_cu;                      /* LDS tier only: cap on resident waves per CU
                                             (0: as many as its occupancy allows)       */
   int queue_chunk;                       /* LDS tier pairs per queue grab when the
-                                            tuning's queue_chunk[0] is 0 (0: 4)        */
+                                            tuning's queue_chunk[0] is 0 (0: 2)        */
   int distance_metric;                   /* bw_distance_metric_t (default BW_GAP_AFFINE) */
   bw_affine2p_penalties_t affine2p_penalties;   /* BW_GAP_AFFINE_2P: default {0,4,6,2,24,1} */
 } bw_attr_t;                             /* wavefront_attributes.h wavefront_aligner_attr_t */
